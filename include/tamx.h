@@ -1,0 +1,300 @@
+/*
+ * tamx.h -- C ABI of the MI355X-native TiledArray contraction hot path.
+ *
+ * Plain C types only: device pointers, sizes and an opaque context.  A stream
+ * argument is a hipStream_t passed as void* (NULL = the default stream).
+ * Every entry point returns a tamx_status; tamx_last_error() returns a
+ * thread-local message describing the last failure.
+ *
+ * Conventions shared with TiledArray:
+ *   - tiles are dense row-major blocks (TA::Tensor storage, tensor/tensor.h:454-458);
+ *   - GEMM is row-major C[m x n] = alpha*op(A)*op(B) + beta*C with
+ *     lda = NoTrans ? k : m, ldb = NoTrans ? n : k, ldc = n
+ *     (tensor/kernels.h:146-158,226; math/blas.h:171-177);
+ *   - permutations use the "TO" convention of permutation.h:115-150:
+ *     result extent[perm[d]] = source extent[d];
+ *   - a tile is zero when its per-element (volume-scaled) Frobenius norm is
+ *     below the shape threshold (sparse_shape.h:150-217, :495-499).
+ *   The shape threshold is always an explicit argument, never a global.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef TAMX_H
+#define TAMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status --- */
+
+typedef enum tamx_status {
+  TAMX_OK = 0,
+  TAMX_ERR_BAD_ARG = 1,     /* replaces TA_ASSERT failures on arguments */
+  TAMX_ERR_OOM = 2,         /* device allocation failed */
+  TAMX_ERR_UNSUPPORTED = 3, /* shape/rank/precision this build does not handle */
+  TAMX_ERR_DEVICE = 4,      /* a HIP runtime call failed */
+  TAMX_ERR_CAPACITY = 5     /* a work list or output pool was too small */
+} tamx_status;
+
+typedef enum tamx_op { TAMX_NO_TRANS = 0, TAMX_TRANS = 1 } tamx_op;
+
+/* Arithmetic of a grouped GEMM launch. */
+typedef enum tamx_precision {
+  TAMX_PREC_F64 = 0,    /* FP64 MFMA (v_mfma_f64_16x16x4) */
+  TAMX_PREC_F32 = 1,    /* FP32 MFMA on FP32 data (tensor-of-tensor FP32) */
+  TAMX_PREC_BF16X3 = 2, /* FP64 data split into 3 BF16 limbs */
+  TAMX_PREC_BF16XS = 3, /* FP64 data split into nslices BF16 limbs */
+  TAMX_PREC_F32X2 = 4   /* FP64 data split into 2 FP32 limbs */
+} tamx_precision;
+
+const char* tamx_status_string(int status);
+const char* tamx_last_error(void);
+/* Library version string ("tamx <semver> gfx950"). */
+const char* tamx_version(void);
+
+/* ----------------------------------------------------------- context --- */
+
+typedef struct tamx_context tamx_context;
+
+/* Replaces the device part of TA::initialize (tiledarray.cpp:32-57,106):
+ * binds the calling thread's work to HIP device `device`. */
+int tamx_context_create(int device, tamx_context** ctx);
+/* Replaces TA::finalize's device teardown (tiledarray.cpp:167-186). */
+int tamx_context_destroy(tamx_context* ctx);
+/* Number of compute units of the context's device. */
+int tamx_context_num_cus(const tamx_context* ctx);
+
+/* --------------------------------------------------- tile-pair work --- */
+
+/* One result tile of a grouped GEMM: C tile = alpha * sum_p op(A_p)op(B_p)
+ * (+ C tile when beta = 1).  Mirrors one ReducePairTask
+ * (reduce_task.h:961-1026): the first contribution of ContractReduce has
+ * beta = 0 on an empty result, later ones beta = 1
+ * (tile_op/contract_reduce.h:409-453). */
+typedef struct tamx_gemm_task {
+  int64_t c_off;      /* element offset of the result tile in C */
+  int32_t m, n;       /* result tile extents (fused rows, fused columns) */
+  int32_t ldc;        /* leading dimension of the stored tile */
+  int32_t pair_begin; /* first contributing pair in the pair array */
+  int32_t pair_count; /* number of contributing pairs */
+  int32_t flags;      /* TAMX_TASK_* bits */
+} tamx_gemm_task;
+
+enum {
+  TAMX_TASK_BETA1 = 1,       /* accumulate into the existing C tile */
+  TAMX_TASK_STORE_TRANS = 2  /* store C^T (n x m, ldc >= m): fused result
+                                permutation that is a matrix transpose of
+                                the (left-outer, right-outer) result,
+                                contract_reduce.h:370-378 */
+};
+
+/* One tile pair (A_ik, B_kj) contributing to a task. */
+typedef struct tamx_gemm_pair {
+  int64_t a_off; /* element offset of A_ik (storage k x m when op_a = TRANS) */
+  int64_t b_off; /* element offset of B_kj (storage n x k when op_b = TRANS) */
+  int32_t k;     /* contracted extent */
+  int32_t lda, ldb;
+  int32_t reserved;
+} tamx_gemm_pair;
+
+/* Device-resident work list of tasks and pairs with fixed capacity, so that
+ * building and consuming it needs no host synchronisation and can be
+ * captured in a hipGraph. */
+typedef struct tamx_worklist tamx_worklist;
+
+/* Layout of the int64 counters the work list keeps on the device. */
+enum {
+  TAMX_WL_NTASKS = 0,  /* number of tasks */
+  TAMX_WL_NPAIRS = 1,  /* number of pairs */
+  TAMX_WL_CELEMS = 2,  /* result elements allocated in C */
+  TAMX_WL_STATUS = 3,  /* 0, or a TAMX_ERR_CAPACITY overflow mark */
+  TAMX_WL_FLOPS = 4,   /* sum over pairs of 2*m*n*k */
+  TAMX_WL_NCOUNTERS = 8
+};
+
+int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
+                         int64_t pair_capacity, int64_t grid_capacity,
+                         tamx_worklist** wl);
+int tamx_worklist_destroy(tamx_worklist* wl);
+/* Device pointers of the work list, for callers that fill tasks and pairs
+ * themselves (e.g. a ContractReduce-compatible op recording pairs).  Such a
+ * caller writes counters[TAMX_WL_NTASKS] and counters[TAMX_WL_NPAIRS]. */
+int tamx_worklist_buffers(tamx_worklist* wl, tamx_gemm_task** tasks,
+                          tamx_gemm_pair** pairs, int64_t** counters);
+/* Copies the counters to host memory (synchronises `stream`). */
+int tamx_worklist_read_counters(tamx_worklist* wl, int64_t* host_counters,
+                                void* stream);
+
+/* ---------------------------------------------------------- planner --- */
+
+#define TAMX_PLAN_MAX_RANK 8
+#define TAMX_PLAN_NAME_LEN 64
+
+/* Host-side plan of C(target) = A(left) * B(right): the decisions of
+ * ContEngine::init_indices/init_struct (expressions/cont_engine.h:354-470)
+ * and GEMMPermutationOptimizer (expressions/permopt.h:254-381). */
+typedef struct tamx_contraction_plan {
+  int32_t left_rank, right_rank, result_rank, inner_rank;
+  int32_t left_permtype, right_permtype; /* 1 identity, 2 matrix_transpose,
+                                            3 general (PermutationType) */
+  int32_t op_a, op_b;                    /* to_cblas_op of the permtypes */
+  int32_t left_perm[TAMX_PLAN_MAX_RANK];  /* TO permutation applied to each
+                                             left tile (identity unless
+                                             general) */
+  int32_t right_perm[TAMX_PLAN_MAX_RANK];
+  int32_t result_perm[TAMX_PLAN_MAX_RANK]; /* natural result order ->
+                                              target, TO convention */
+  int32_t result_perm_identity;
+  int32_t result_perm_matrix_transpose;  /* result permute == transpose of
+                                            the fused result tile */
+  int32_t left_gemm_is_inner[TAMX_PLAN_MAX_RANK];  /* per mode of the left
+                                                      GEMM layout */
+  int32_t right_gemm_is_inner[TAMX_PLAN_MAX_RANK];
+  char left_target[TAMX_PLAN_NAME_LEN];   /* left index order fed to GEMM */
+  char right_target[TAMX_PLAN_NAME_LEN];
+  char result_natural[TAMX_PLAN_NAME_LEN]; /* index order GEMM produces */
+} tamx_contraction_plan;
+
+/* Annotations are comma-separated index names ("a,b,i,j").
+ * prefer_to_permute_left mirrors `left_type::leaves <= right_type::leaves`
+ * (expressions/binary_engine.h:120-123), 1 for two leaf arrays. */
+int tamx_plan_contraction(const char* left, const char* right,
+                          const char* target, int prefer_to_permute_left,
+                          tamx_contraction_plan* plan);
+
+/* ------------------------------------------------------- screening --- */
+
+/* Per-tile Frobenius norms, cast to float.  Replaces Tensor::norm()
+ * (tensor/tensor.h:3768-3787) as called by make_array for sparse arrays
+ * (conversions/make_array.h:152-205).  A plan fixes the tile table so a
+ * launch needs no host data; tiles with offset < 0 get norm 0. */
+typedef struct tamx_norm_plan tamx_norm_plan;
+int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
+                          const int64_t* host_volumes, int64_t ntiles,
+                          tamx_norm_plan** plan);
+int tamx_norm_plan_destroy(tamx_norm_plan* plan);
+int tamx_tile_norms_f64(tamx_context* ctx, const tamx_norm_plan* plan,
+                        const double* pool, float* norms, void* stream);
+int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
+                        const float* pool, float* norms, void* stream);
+
+/* Scales raw tile norms in place by inverse tile volume and screens them.
+ * Replaces SparseShape's norm constructor (sparse_shape.h:335-349, scaling
+ * :150-217).  norms: row-major tile grid of `rank` modes (nbatch grids back
+ * to back); host_ntiles[d]: tiles of mode d; sizes: device floats, the tile
+ * extents of mode 0, then mode 1, ... (per batch when sizes_per_batch != 0).
+ * zero_count (device int64, may be NULL) receives the number of zeroed
+ * tiles (added atomically). */
+int tamx_shape_scale(tamx_context* ctx, float* norms, int rank,
+                     const int64_t* host_ntiles, int64_t nbatch,
+                     const float* sizes, int sizes_per_batch, float threshold,
+                     int64_t* zero_count, void* stream);
+
+/* Result shape estimate of a contraction.  Replaces SparseShape::gemm
+ * (sparse_shape.h:1589-1681): left is the M x K tile-norm matrix in storage
+ * order (K x M if op_a), right K x N (N x K if op_b), k_sizes the K volumes
+ * of the contracted tiles.  est = abs_factor * sum_k l*r computed in float,
+ * k ascending, products and sums rounded separately; est < threshold -> 0.
+ * nbatch independent problems may be stacked (batch strides M*K, K*N, K,
+ * M*N).  zero_count as above. */
+int tamx_shape_gemm_f32(tamx_context* ctx, const float* left,
+                        const float* right, int op_a, int op_b, int64_t M,
+                        int64_t N, int64_t K, int64_t nbatch,
+                        const float* k_sizes, float abs_factor,
+                        float threshold, float* result, int64_t* zero_count,
+                        void* stream);
+
+/* A block-sparse contraction on one rank, fused to matrix form.  Device
+ * arrays; nbatch independent problems of the same tile-grid size may be
+ * stacked (the batched product of tile_op/batched_contract_reduce.h). */
+typedef struct tamx_contraction_desc {
+  int32_t op_a, op_b;      /* tamx_op of the fused operands */
+  int64_t Mt, Nt, Kt;      /* tile-grid extents of one problem */
+  int64_t nbatch;          /* >= 1 */
+  const int32_t* mext;     /* [nbatch*Mt] fused row extent of row tile i */
+  const int32_t* next;     /* [nbatch*Nt] */
+  const int32_t* kext;     /* [nbatch*Kt] */
+  const int64_t* a_off;    /* [nbatch*Mt*Kt] tile offsets in A, < 0 absent
+                              (grid in storage order: Kt x Mt if op_a) */
+  const int64_t* b_off;    /* [nbatch*Kt*Nt] (Nt x Kt if op_b) */
+  const float* a_norms;    /* optional scaled norms (same grid as a_off) */
+  const float* b_norms;    /* optional */
+  const float* c_norms;    /* [nbatch*Mt*Nt] result shape estimate */
+  float a_threshold;       /* tile present iff offset >= 0 and norm >= thr */
+  float b_threshold;
+  float c_threshold;       /* result tile non-zero iff c_norm >= thr */
+  float pair_threshold;    /* > 0: skip pairs with a_norm*b_norm <
+                              pair_threshold/Kt, the per-pair rule the
+                              reference compiles out
+                              (contraction_eval.h:1386-1441); 0: off */
+  int32_t store_trans;     /* 1: result tiles stored transposed */
+  int64_t* c_off;          /* out [nbatch*Mt*Nt]: offsets in C, -1 zero */
+  int64_t c_capacity;      /* elements available in C */
+} tamx_contraction_desc;
+
+/* Builds the work list of a contraction on the device: one task per non-zero
+ * result tile, pairs for every k (ascending) with both operands non-zero.
+ * Replaces the single-rank schedule of Summa (contraction_eval.h:1094-1161
+ * initialize, :974-1001 iterate_sparse, :1348-1384 contract) and the
+ * allocation of result tiles by ReducePairTask.  Result tiles are packed in
+ * C in row-major tile order. */
+int tamx_contraction_worklist(tamx_context* ctx,
+                              const tamx_contraction_desc* desc,
+                              tamx_worklist* wl, void* stream);
+
+/* -------------------------------------------------------- permute --- */
+
+/* Permutes many tiles of one rank with one permutation in one launch.
+ * Replaces per-tile Tensor(other, perm) / detail::permute
+ * (tensor/tensor.h:644-692, tensor/permute.h:121-207) reached from the
+ * permute-on-read path (dist_eval/array_eval.h:330-365 ->
+ * tile_interface/permute.h:74-134) and the result permute of
+ * ContractReduce (contract_reduce.h:370-378); also replaces the per-call
+ * LibreTT plan of the device build (external/librett.h:79-111).
+ * perm: "TO" convention, result extent[perm[d]] = extent[d].
+ * host_extents: ntiles x rank source extents (row-major tiles);
+ * host_src_off / host_dst_off: element offsets of each tile in the source
+ * and result pools.  The plan keeps the tile table on the device. */
+typedef struct tamx_permute_plan tamx_permute_plan;
+int tamx_permute_plan_create(tamx_context* ctx, int rank, const int32_t* perm,
+                             int64_t ntiles, const int64_t* host_extents,
+                             const int64_t* host_src_off,
+                             const int64_t* host_dst_off,
+                             tamx_permute_plan** plan);
+int tamx_permute_plan_destroy(tamx_permute_plan* plan);
+/* Elements moved by one execution (each read once, written once). */
+int64_t tamx_permute_plan_volume(const tamx_permute_plan* plan);
+int tamx_permute_f64(tamx_context* ctx, const tamx_permute_plan* plan,
+                     const double* src, double* dst, void* stream);
+int tamx_permute_f32(tamx_context* ctx, const tamx_permute_plan* plan,
+                     const float* src, float* dst, void* stream);
+
+/* ---------------------------------------------------- grouped GEMM --- */
+
+/* All tasks of a work list in one persistent launch.  Replaces the per-pair
+ * gemm customisation points (tile_op/tile_interface.h:805-830) that
+ * ContractReduce calls (contract_reduce.h:446-451), reaching
+ * Tensor::gemm -> detail::gemm -> math::blas::gemm (tensor.h:3131-3219,
+ * kernels.h:90-231, blas.h:171-177). */
+int tamx_grouped_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                          int op_b, double alpha, const double* A,
+                          const double* B, double* C, void* stream);
+/* Same, with the BM x BN macro tile chosen by the caller (an autotuner):
+ * (64,64), (128,128), (128,64), (64,128) or (32,32). */
+int tamx_grouped_gemm_f64_tiled(tamx_context* ctx, tamx_worklist* wl,
+                                int op_a, int op_b, double alpha,
+                                const double* A, const double* B, double* C,
+                                int bm, int bn, void* stream);
+/* FP32 data, FP32 MFMA: tensor-of-tensor cells in single precision. */
+int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                          int op_b, float alpha, const float* A,
+                          const float* B, float* C, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TAMX_H */

@@ -1,0 +1,517 @@
+// Sparse-shape screening on the device: tile norms, inverse-volume scaling,
+// the float norm-GEMM result estimate, and the work-list builder that turns
+// the screened shapes into the tasks/pairs of a grouped GEMM.
+//
+// Float arithmetic that decides a sparsity pattern is written with explicit
+// round-to-nearest intrinsics (__fmul_rn, __fadd_rn, __fdiv_rn) so no
+// multiply-add is ever contracted and every step rounds exactly where the
+// reference's expression does; the pattern is then reproducible bit for bit.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "tamx_internal.hpp"
+#include "tamx_scan.hpp"
+
+struct tamx_norm_plan {
+  tamx_context* ctx = nullptr;
+  int64_t ntiles = 0;
+  int64_t nchunks = 0;
+  // chunk c covers elements [cbeg[c], cend[c]) of pool offsets of tile ctile[c]
+  int64_t* cbeg = nullptr;
+  int64_t* cend = nullptr;
+  int32_t* ctile = nullptr;
+  // tile t owns chunks [tchunk[t], tchunk[t+1]); no chunk for absent tiles
+  int64_t* tchunk = nullptr;
+  double* partial = nullptr;  // one partial sum per chunk
+};
+
+namespace {
+
+constexpr int64_t kNormChunk = 1 << 16;  // elements per workgroup
+constexpr int kNormThreads = 256;
+
+// Partial sums of squares, one workgroup per chunk.  Accumulation in the
+// tile's scalar type: double for FP64 tiles, float for FP32 tiles, as
+// Tensor<T>::squared_norm does (tensor/tensor.h:3768-3776).
+template <typename T>
+__global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
+    const T* __restrict__ pool, const int64_t* __restrict__ cbeg,
+    const int64_t* __restrict__ cend, double* __restrict__ partial) {
+  __shared__ T red[kNormThreads];
+  const int64_t c = blockIdx.x;
+  const int64_t b = cbeg[c], e = cend[c];
+  T s = T(0);
+  for (int64_t i = b + threadIdx.x; i < e; i += kNormThreads) {
+    const T x = pool[i];
+    s += x * x;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kNormThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[c] = (double)red[0];
+}
+
+// Combines the chunks of each tile in chunk order: norm = sqrt(sum), cast to
+// the float shape type (make_array stores into Tensor<float>,
+// conversions/make_array.h:170-172).
+template <typename T>
+__global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
+                                   const double* __restrict__ partial,
+                                   int64_t ntiles, float* __restrict__ norms) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int64_t b = tchunk[t], e = tchunk[t + 1];
+  if (b == e) {
+    norms[t] = 0.0f;
+    return;
+  }
+  if (sizeof(T) == sizeof(double)) {
+    double s = 0.0;
+    for (int64_t c = b; c < e; ++c) s += partial[c];
+    norms[t] = (float)sqrt(s);
+  } else {
+    float s = 0.0f;
+    for (int64_t c = b; c < e; ++c) s += (float)partial[c];
+    norms[t] = sqrtf(s);
+  }
+}
+
+template <typename T>
+int tile_norms(tamx_context* ctx, const tamx_norm_plan* plan, const T* pool,
+               float* norms, void* stream) {
+  TAMX_REQUIRE(plan != nullptr && norms != nullptr, "null argument");
+  if (plan->ntiles == 0) return TAMX_OK;
+  TAMX_REQUIRE(pool != nullptr || plan->nchunks == 0, "null pool");
+  if (int rc = tamx::activate(ctx)) return rc;
+  hipStream_t s = tamx::as_stream(stream);
+  if (plan->nchunks > 0) {
+    norm_partial_kernel<T><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
+        pool, plan->cbeg, plan->cend, plan->partial);
+    TAMX_LAUNCH_CHECK();
+  }
+  const unsigned grid = (unsigned)((plan->ntiles + 255) / 256);
+  norm_finish_kernel<T><<<grid, 256, 0, s>>>(plan->tchunk, plan->partial,
+                                              plan->ntiles, norms);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+// ------------------------------------------------------- shape scaling --
+
+constexpr int kMaxRank = 8;
+
+struct ScaleParams {
+  float* norms;
+  const float* sizes;
+  int64_t* zero_count;
+  int64_t ntiles[kMaxRank];
+  int64_t mode_off[kMaxRank];  // start of each mode in `sizes`
+  int64_t per_batch;           // tiles of one grid
+  int64_t sizes_batch_stride;  // 0 when sizes are shared by all batches
+  int64_t nbatch;
+  int rank;
+  float threshold;
+};
+
+// recursive_outer_product with the inverse-volume op (sparse_shape.h:106-131,
+// :177-181) evaluated for one multi-index: modes [lo, hi).
+__device__ float inv_rop(const ScaleParams& p, const float* sizes,
+                         const int64_t* idx, int lo, int hi) {
+  if (hi - lo == 1) return __fdiv_rn(1.0f, sizes[p.mode_off[lo] + idx[lo]]);
+  const int n = hi - lo;
+  const int middle = (n >> 1) + (n & 1);
+  const float l = inv_rop(p, sizes, idx, lo, lo + middle);
+  const float r = inv_rop(p, sizes, idx, lo + middle, hi);
+  return __fmul_rn(l, r);
+}
+
+__global__ void shape_scale_kernel(ScaleParams p) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p.per_batch * p.nbatch) return;
+  const int64_t b = g / p.per_batch;
+  int64_t ord = g % p.per_batch;
+  const float* sizes = p.sizes + b * p.sizes_batch_stride;
+  int64_t idx[kMaxRank];
+  for (int d = p.rank - 1; d >= 0; --d) {
+    idx[d] = ord % p.ntiles[d];
+    ord /= p.ntiles[d];
+  }
+  float v = p.norms[g];
+  if (p.rank == 1) {
+    v = __fdiv_rn(v, sizes[idx[0]]);  // rank-1 divides (sparse_shape.h:160-173)
+  } else {
+    const int middle = (p.rank >> 1) + (p.rank & 1);
+    const float x = inv_rop(p, sizes, idx, 0, middle);
+    const float y = inv_rop(p, sizes, idx, middle, p.rank);
+    v = __fmul_rn(v, __fmul_rn(x, y));  // norm *= x * y (:205-213)
+  }
+  if (v < p.threshold) {
+    v = 0.0f;
+    if (p.zero_count) atomicAdd((unsigned long long*)p.zero_count, 1ull);
+  }
+  p.norms[g] = v;
+}
+
+// ---------------------------------------------------------- shape GEMM --
+
+struct ShapeGemmParams {
+  const float* left;
+  const float* right;
+  const float* ksz;
+  float* result;
+  int64_t* zero_count;
+  int64_t M, N, K, nbatch;
+  int op_a, op_b;
+  float abs_factor, threshold;
+};
+
+// est_ij = abs_factor * sum_k l_ik * r_kj with the k_sizes scaling applied
+// along contiguous storage runs as SparseShape::gemm does
+// (sparse_shape.h:1632-1647): left[s] *= k_sizes[s % K], right[s] *=
+// k_sizes[s / N].  One thread per result tile, k ascending.
+__global__ void shape_gemm_kernel(ShapeGemmParams p) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p.M * p.N * p.nbatch) return;
+  const int64_t b = g / (p.M * p.N);
+  const int64_t ij = g % (p.M * p.N);
+  const int64_t i = ij / p.N, j = ij % p.N;
+  const float* L = p.left + b * p.M * p.K;
+  const float* R = p.right + b * p.K * p.N;
+  const float* ks = p.ksz + b * p.K;
+  float v;
+  if (p.K == 0) {
+    v = __fmul_rn(__fmul_rn(L[i], R[j]), p.abs_factor);  // outer product (:1666-1678)
+  } else {
+    float acc = 0.0f;
+    for (int64_t k = 0; k < p.K; ++k) {
+      const int64_t sl = p.op_a ? k * p.M + i : i * p.K + k;
+      const int64_t sr = p.op_b ? j * p.K + k : k * p.N + j;
+      const float l = __fmul_rn(L[sl], ks[sl % p.K]);
+      const float r = __fmul_rn(R[sr], ks[sr / p.N]);
+      acc = __fadd_rn(acc, __fmul_rn(l, r));
+    }
+    v = __fmul_rn(p.abs_factor, acc);
+  }
+  if (v < p.threshold) {
+    v = 0.0f;
+    if (p.zero_count) atomicAdd((unsigned long long*)p.zero_count, 1ull);
+  }
+  p.result[g] = v;
+}
+
+// ---------------------------------------------------- work-list builder --
+
+struct WlParams {
+  tamx_contraction_desc d;
+  tamx_gemm_task* tasks;
+  tamx_gemm_pair* pairs;
+  int64_t* counters;
+  int32_t* tile_npairs;
+  int64_t* tile_volume;
+  int64_t* tile_flops;
+  int32_t* tile_task;
+  int32_t* tile_pair_begin;
+  int64_t* tile_c_begin;
+  int64_t task_capacity, pair_capacity;
+};
+
+__device__ inline bool a_present(const tamx_contraction_desc& d, int64_t ai) {
+  return d.a_off[ai] >= 0 && (!d.a_norms || !(d.a_norms[ai] < d.a_threshold));
+}
+__device__ inline bool b_present(const tamx_contraction_desc& d, int64_t bi) {
+  return d.b_off[bi] >= 0 && (!d.b_norms || !(d.b_norms[bi] < d.b_threshold));
+}
+__device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
+                                 int64_t bi, float thr_k) {
+  if (!a_present(d, ai) || !b_present(d, bi)) return false;
+  if (thr_k > 0.0f && d.a_norms && d.b_norms &&
+      __fmul_rn(d.a_norms[ai], d.b_norms[bi]) < thr_k)
+    return false;
+  return true;
+}
+
+// Pass 1: per result tile, the number of contributing pairs, the element
+// count and the flops (zero for result tiles screened out by the shape).
+__global__ void wl_count_kernel(WlParams p) {
+  const tamx_contraction_desc& d = p.d;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = d.Mt * d.Nt;
+  if (g >= per * d.nbatch) return;
+  const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
+  int32_t np = 0;
+  int64_t vol = 0, flops = 0;
+  if (!(d.c_norms[g] < d.c_threshold)) {
+    const int64_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
+    vol = m * n;
+    const float thr_k = d.pair_threshold > 0.0f
+                            ? __fdiv_rn(d.pair_threshold, (float)d.Kt)
+                            : 0.0f;
+    for (int64_t k = 0; k < d.Kt; ++k) {
+      const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+      const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
+      if (pair_kept(d, ai, bi, thr_k)) {
+        ++np;
+        flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k];
+      }
+    }
+  }
+  p.tile_npairs[g] = np;
+  p.tile_volume[g] = vol;
+  p.tile_flops[g] = flops;
+}
+
+// Pass 2 (one workgroup): exclusive scans give each non-zero result tile its
+// task index, first pair and place in C (row-major tile order).
+__global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p) {
+  __shared__ int64_t part64[1024];
+  const tamx_contraction_desc& d = p.d;
+  const int64_t n = d.Mt * d.Nt * d.nbatch;
+  const int64_t ntasks = tamx::block_exclusive_scan<1024, int64_t>(
+      n, part64, [&](int64_t i) -> int64_t { return p.tile_volume[i] > 0 ? 1 : 0; },
+      [&](int64_t i, int64_t v) { p.tile_task[i] = (int32_t)v; });
+  const int64_t npairs = tamx::block_exclusive_scan<1024, int64_t>(
+      n, part64, [&](int64_t i) -> int64_t { return p.tile_npairs[i]; },
+      [&](int64_t i, int64_t v) { p.tile_pair_begin[i] = (int32_t)v; });
+  const int64_t celems = tamx::block_exclusive_scan<1024, int64_t>(
+      n, part64, [&](int64_t i) -> int64_t { return p.tile_volume[i]; },
+      [&](int64_t i, int64_t v) { p.tile_c_begin[i] = v; });
+  const int64_t flops = tamx::block_exclusive_scan<1024, int64_t>(
+      n, part64, [&](int64_t i) -> int64_t { return p.tile_flops[i]; },
+      [&](int64_t, int64_t) {});
+  if (threadIdx.x == 0) {
+    int64_t status = 0;
+    if (ntasks > p.task_capacity || npairs > p.pair_capacity ||
+        celems > d.c_capacity)
+      status = TAMX_ERR_CAPACITY;
+    p.counters[TAMX_WL_NTASKS] = status ? 0 : ntasks;
+    p.counters[TAMX_WL_NPAIRS] = status ? 0 : npairs;
+    p.counters[TAMX_WL_CELEMS] = celems;
+    p.counters[TAMX_WL_STATUS] = status;
+    p.counters[TAMX_WL_FLOPS] = flops;
+    // record the totals that did not fit, for the host's error message
+    p.counters[5] = ntasks;
+    p.counters[6] = npairs;
+    p.counters[7] = 0;
+  }
+}
+
+// Pass 3: write tasks, pairs (k ascending) and the C offsets.
+__global__ void wl_fill_kernel(WlParams p) {
+  const tamx_contraction_desc& d = p.d;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = d.Mt * d.Nt;
+  if (g >= per * d.nbatch) return;
+  const bool ok = p.counters[TAMX_WL_STATUS] == 0;
+  if (!ok || p.tile_volume[g] == 0) {
+    d.c_off[g] = -1;
+    return;
+  }
+  const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
+  const int32_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
+  const int64_t coff = p.tile_c_begin[g];
+  d.c_off[g] = coff;
+  const int32_t t = p.tile_task[g];
+  const int32_t pb = p.tile_pair_begin[g];
+  tamx_gemm_task task;
+  task.c_off = coff;
+  task.m = m;
+  task.n = n;
+  task.ldc = d.store_trans ? m : n;
+  task.pair_begin = pb;
+  task.pair_count = p.tile_npairs[g];
+  task.flags = d.store_trans ? TAMX_TASK_STORE_TRANS : 0;
+  p.tasks[t] = task;
+  const float thr_k =
+      d.pair_threshold > 0.0f ? __fdiv_rn(d.pair_threshold, (float)d.Kt) : 0.0f;
+  int32_t q = pb;
+  for (int64_t k = 0; k < d.Kt; ++k) {
+    const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+    const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
+    if (!pair_kept(d, ai, bi, thr_k)) continue;
+    const int32_t kk = d.kext[b * d.Kt + k];
+    tamx_gemm_pair pr;
+    pr.a_off = d.a_off[ai];
+    pr.b_off = d.b_off[bi];
+    pr.k = kk;
+    pr.lda = d.op_a ? m : kk;
+    pr.ldb = d.op_b ? kk : n;
+    pr.reserved = 0;
+    p.pairs[q++] = pr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
+                          const int64_t* host_volumes, int64_t ntiles,
+                          tamx_norm_plan** out) {
+  TAMX_REQUIRE(out != nullptr, "null output pointer");
+  *out = nullptr;
+  TAMX_REQUIRE(ntiles >= 0, "negative tile count");
+  TAMX_REQUIRE(ntiles == 0 || (host_offsets && host_volumes), "null tile table");
+  if (int rc = tamx::activate(ctx)) return rc;
+  std::vector<int64_t> cbeg, cend, tchunk(ntiles + 1);
+  std::vector<int32_t> ctile;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    tchunk[t] = (int64_t)cbeg.size();
+    if (host_offsets[t] < 0) continue;
+    TAMX_REQUIRE(host_volumes[t] >= 0, "negative tile volume");
+    for (int64_t b = 0; b < host_volumes[t]; b += kNormChunk) {
+      cbeg.push_back(host_offsets[t] + b);
+      cend.push_back(host_offsets[t] + std::min(host_volumes[t], b + kNormChunk));
+      ctile.push_back((int32_t)t);
+    }
+  }
+  tchunk[ntiles] = (int64_t)cbeg.size();
+  auto* plan = new (std::nothrow) tamx_norm_plan();
+  if (!plan) return TAMX_ERR_OOM;
+  plan->ctx = ctx;
+  plan->ntiles = ntiles;
+  plan->nchunks = (int64_t)cbeg.size();
+  const size_t nc = std::max<size_t>(1, cbeg.size());
+  hipError_t e = hipSuccess;
+  auto alloc = [&](auto** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  };
+  alloc(&plan->cbeg, nc * sizeof(int64_t));
+  alloc(&plan->cend, nc * sizeof(int64_t));
+  alloc(&plan->ctile, nc * sizeof(int32_t));
+  alloc(&plan->partial, nc * sizeof(double));
+  alloc(&plan->tchunk, (ntiles + 1) * sizeof(int64_t));
+  if (e == hipSuccess && !cbeg.empty()) {
+    e = hipMemcpy(plan->cbeg, cbeg.data(), cbeg.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(plan->cend, cend.data(), cend.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(plan->ctile, ctile.data(), ctile.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->tchunk, tchunk.data(), tchunk.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tamx::set_error(std::string("norm plan: ") + hipGetErrorString(e));
+    tamx_norm_plan_destroy(plan);
+    return e == hipErrorOutOfMemory ? TAMX_ERR_OOM : TAMX_ERR_DEVICE;
+  }
+  *out = plan;
+  return TAMX_OK;
+}
+
+int tamx_norm_plan_destroy(tamx_norm_plan* plan) {
+  if (!plan) return TAMX_OK;
+  void* ptrs[] = {plan->cbeg, plan->cend, plan->ctile, plan->partial, plan->tchunk};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete plan;
+  return TAMX_OK;
+}
+
+int tamx_tile_norms_f64(tamx_context* ctx, const tamx_norm_plan* plan,
+                        const double* pool, float* norms, void* stream) {
+  return tile_norms<double>(ctx, plan, pool, norms, stream);
+}
+
+int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
+                        const float* pool, float* norms, void* stream) {
+  return tile_norms<float>(ctx, plan, pool, norms, stream);
+}
+
+int tamx_shape_scale(tamx_context* ctx, float* norms, int rank,
+                     const int64_t* host_ntiles, int64_t nbatch,
+                     const float* sizes, int sizes_per_batch, float threshold,
+                     int64_t* zero_count, void* stream) {
+  TAMX_REQUIRE(rank >= 1 && rank <= kMaxRank, "rank must be in [1, 8]");
+  TAMX_REQUIRE(host_ntiles != nullptr && norms != nullptr && sizes != nullptr,
+               "null argument");
+  TAMX_REQUIRE(nbatch >= 1, "nbatch must be >= 1");
+  if (int rc = tamx::activate(ctx)) return rc;
+  ScaleParams p{};
+  p.norms = norms;
+  p.sizes = sizes;
+  p.zero_count = zero_count;
+  p.rank = rank;
+  p.threshold = threshold;
+  p.nbatch = nbatch;
+  int64_t per = 1, off = 0;
+  for (int d = 0; d < rank; ++d) {
+    TAMX_REQUIRE(host_ntiles[d] >= 0, "negative tile count");
+    p.ntiles[d] = host_ntiles[d];
+    p.mode_off[d] = off;
+    off += host_ntiles[d];
+    per *= host_ntiles[d];
+  }
+  p.per_batch = per;
+  p.sizes_batch_stride = sizes_per_batch ? off : 0;
+  const int64_t n = per * nbatch;
+  if (n == 0) return TAMX_OK;
+  shape_scale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, tamx::as_stream(stream)>>>(p);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+int tamx_shape_gemm_f32(tamx_context* ctx, const float* left,
+                        const float* right, int op_a, int op_b, int64_t M,
+                        int64_t N, int64_t K, int64_t nbatch,
+                        const float* k_sizes, float abs_factor,
+                        float threshold, float* result, int64_t* zero_count,
+                        void* stream) {
+  TAMX_REQUIRE(M >= 0 && N >= 0 && K >= 0 && nbatch >= 1, "bad sizes");
+  TAMX_REQUIRE(left && right && result && (K == 0 || k_sizes), "null argument");
+  TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be 0 or 1");
+  TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be 0 or 1");
+  if (int rc = tamx::activate(ctx)) return rc;
+  ShapeGemmParams p{left, right, k_sizes, result, zero_count, M, N, K, nbatch,
+                    op_a, op_b, abs_factor, threshold};
+  const int64_t n = M * N * nbatch;
+  if (n == 0) return TAMX_OK;
+  shape_gemm_kernel<<<(unsigned)((n + 255) / 256), 256, 0, tamx::as_stream(stream)>>>(p);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+int tamx_contraction_worklist(tamx_context* ctx,
+                              const tamx_contraction_desc* desc,
+                              tamx_worklist* wl, void* stream) {
+  TAMX_REQUIRE(desc != nullptr && wl != nullptr, "null argument");
+  const tamx_contraction_desc& d = *desc;
+  TAMX_REQUIRE(d.op_a == 0 || d.op_a == 1, "op_a must be 0 or 1");
+  TAMX_REQUIRE(d.op_b == 0 || d.op_b == 1, "op_b must be 0 or 1");
+  TAMX_REQUIRE(d.Mt >= 0 && d.Nt >= 0 && d.Kt >= 0 && d.nbatch >= 1, "bad grid");
+  TAMX_REQUIRE(d.mext && d.next && d.kext && d.a_off && d.b_off && d.c_norms &&
+                   d.c_off,
+               "null descriptor array");
+  const int64_t grid = d.Mt * d.Nt * d.nbatch;
+  if (grid > wl->grid_capacity) {
+    tamx::set_error("result tile grid exceeds the work list's grid capacity");
+    return TAMX_ERR_CAPACITY;
+  }
+  if (int rc = tamx::activate(ctx)) return rc;
+  WlParams p{d,
+             wl->tasks,
+             wl->pairs,
+             wl->counters,
+             wl->tile_npairs,
+             wl->tile_volume,
+             wl->tile_flops,
+             wl->tile_task,
+             wl->tile_pair_begin,
+             wl->tile_c_begin,
+             wl->task_capacity,
+             wl->pair_capacity};
+  hipStream_t s = tamx::as_stream(stream);
+  const unsigned g = (unsigned)std::max<int64_t>(1, (grid + 255) / 256);
+  wl_count_kernel<<<g, 256, 0, s>>>(p);
+  TAMX_LAUNCH_CHECK();
+  wl_scan_kernel<<<1, 1024, 0, s>>>(p);
+  TAMX_LAUNCH_CHECK();
+  wl_fill_kernel<<<g, 256, 0, s>>>(p);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,413 @@
+"""Single-rank contraction engine over the C ABI of libtamx.
+
+`contract(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,b,c,d")` evaluates
+C(target) = A(left) * B(right) the way TiledArray's expression layer does on
+one rank, with every data-parallel step on the GPU:
+
+  1. plan (tamx_plan_contraction): operand permutation types, GEMM ops, the
+     natural result order and its permutation to the target
+     (expressions/cont_engine.h:354-470, expressions/permopt.h:254-381);
+  2. permute operand tiles whose layout is `general` (permute-on-read,
+     dist_eval/array_eval.h:330-365) with one tamx_permute launch;
+  3. fuse the operands into block matrices (GemmHelper,
+     math/gemm_helper.h:41-278) and estimate the result shape
+     (SparseShape::gemm, sparse_shape.h:1589-1681);
+  4. build the device work list of surviving tile pairs (Summa's one-rank
+     schedule, dist_eval/contraction_eval.h:1094-1384) and run all of them in
+     one persistent grouped GEMM (ReducePairTask + ContractReduce,
+     reduce_task.h:961-1026, tile_op/contract_reduce.h:409-453);
+  5. apply the result permutation, fused into the GEMM store when it is a
+     matrix transpose (contract_reduce.h:370-378).
+
+A `Contraction` object does steps 1-3's host work once; `run()` then
+enqueues only device work, with no host synchronisation, so it can be
+replayed or captured in a hipGraph.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, lib
+from .tiled import DistArray, TiledRange
+
+FLOAT_EPS = float(np.finfo(np.float32).eps)  # SparseShape<float> default threshold
+
+
+class Context:
+    """Owns a tamx_context for one device."""
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("tiledarray_amd needs a GPU: torch.cuda.is_available() is False")
+        self.device = device
+        torch.cuda.set_device(device)
+        p = ctypes.c_void_p()
+        call("tamx_context_create", device, ctypes.byref(p))
+        self.ptr = p
+        self.num_cus = lib().tamx_context_num_cus(p)
+
+    def stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def close(self):
+        if self.ptr:
+            lib().tamx_context_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WorkList:
+    def __init__(self, ctx, task_capacity, pair_capacity, grid_capacity):
+        self.ctx = ctx
+        p = ctypes.c_void_p()
+        call("tamx_worklist_create", ctx.ptr, int(task_capacity), int(pair_capacity),
+             int(grid_capacity), ctypes.byref(p))
+        self.ptr = p
+        self.task_capacity, self.pair_capacity, self.grid_capacity = \
+            int(task_capacity), int(pair_capacity), int(grid_capacity)
+
+    def counters(self):
+        buf = (ctypes.c_int64 * _lib.WL_NCOUNTERS)()
+        call("tamx_worklist_read_counters", self.ptr, buf, self.ctx.stream())
+        return list(buf)
+
+    def buffers(self):
+        t, p, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        call("tamx_worklist_buffers", self.ptr, ctypes.byref(t), ctypes.byref(p), ctypes.byref(c))
+        return t.value, p.value, c.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_worklist_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class NormPlan:
+    def __init__(self, ctx, offsets, volumes):
+        self.ctx = ctx
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        volumes = np.ascontiguousarray(volumes, np.int64)
+        p = ctypes.c_void_p()
+        call("tamx_norm_plan_create", ctx.ptr, offsets.ctypes.data, volumes.ctypes.data,
+             offsets.size, ctypes.byref(p))
+        self.ptr = p
+        self.ntiles = offsets.size
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_norm_plan_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class PermutePlan:
+    def __init__(self, ctx, perm, extents, src_off, dst_off):
+        self.ctx = ctx
+        perm = np.ascontiguousarray(perm, np.int32)
+        extents = np.ascontiguousarray(extents, np.int64)
+        src_off = np.ascontiguousarray(src_off, np.int64)
+        dst_off = np.ascontiguousarray(dst_off, np.int64)
+        p = ctypes.c_void_p()
+        call("tamx_permute_plan_create", ctx.ptr, int(perm.size), perm.ctypes.data,
+             int(src_off.size), extents.ctypes.data, src_off.ctypes.data, dst_off.ctypes.data,
+             ctypes.byref(p))
+        self.ptr = p
+        self.volume = int(lib().tamx_permute_plan_volume(p))
+
+    def run(self, src, dst):
+        f = "tamx_permute_f64" if src.dtype == torch.float64 else "tamx_permute_f32"
+        call(f, self.ctx.ptr, self.ptr, src.data_ptr(), dst.data_ptr(), self.ctx.stream())
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_permute_plan_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------- shapes ------
+
+def tile_norms(ctx, arr, out=None):
+    """Raw per-tile Frobenius norms (float32, device), Tensor::norm()."""
+    plan = getattr(arr, "_norm_plan", None)
+    if plan is None or plan.ctx is not ctx:
+        plan = NormPlan(ctx, arr.offsets, arr.trange.tile_volumes())
+        arr._norm_plan = plan
+    if out is None:
+        out = torch.empty(arr.trange.ntiles, dtype=torch.float32, device=arr.pool.device)
+    f = "tamx_tile_norms_f64" if arr.dtype == torch.float64 else "tamx_tile_norms_f32"
+    call(f, ctx.ptr, plan.ptr, arr.pool.data_ptr(), out.data_ptr(), ctx.stream())
+    return out
+
+
+def _sizes_vector(trange, device):
+    s = np.concatenate([r.extents for r in trange.ranges]).astype(np.float32)
+    return torch.as_tensor(s, device=device)
+
+
+def scale_norms(ctx, norms, trange, threshold, zero_count=None, sizes=None):
+    """In place: norms /= tile volume (reference op order), screen < threshold."""
+    nt = np.ascontiguousarray(trange.tiles_shape, np.int64)
+    if sizes is None:
+        sizes = _sizes_vector(trange, norms.device)
+    call("tamx_shape_scale", ctx.ptr, norms.data_ptr(), trange.rank, nt.ctypes.data, 1,
+         sizes.data_ptr(), 0, ctypes.c_float(threshold), _ptr(zero_count), ctx.stream())
+    return norms
+
+
+def make_sparse(ctx, arr, threshold=FLOAT_EPS):
+    """Attaches a SparseShape to `arr` built from its tile norms (make_array
+    semantics, conversions/make_array.h:152-205): norms, inverse-volume
+    scaling and screening.  Tiles screened to zero stay in the pool but are
+    treated as absent by the contraction."""
+    n = tile_norms(ctx, arr)
+    scale_norms(ctx, n, arr.trange, threshold)
+    arr.shape_norms = n
+    arr.shape_threshold = float(np.float32(threshold))
+    return arr
+
+
+def k_sizes(inner_ranges):
+    """recursive_outer_product of contracted tile extents (float,
+    sparse_shape.h:106-131 with the identity op, used at :1626-1630)."""
+    def rop(vs):
+        if len(vs) == 1:
+            return vs[0].astype(np.float32)
+        mid = (len(vs) >> 1) + (len(vs) & 1)
+        l, r = rop(vs[:mid]), rop(vs[mid:])
+        return (l[:, None] * r[None, :]).reshape(-1).astype(np.float32)
+    return rop([r.extents.astype(np.float32) for r in inner_ranges])
+
+
+def _fused_extents(ranges):
+    e = np.ones(1, np.int64)
+    for r in ranges:
+        e = np.multiply.outer(e, r.extents.astype(np.int64)).reshape(-1)
+    return e
+
+
+def plan_contraction(left, right, target, prefer_to_permute_left=True):
+    p = _lib.ContractionPlan()
+    call("tamx_plan_contraction", left.encode(), right.encode(), target.encode(),
+         1 if prefer_to_permute_left else 0, ctypes.byref(p))
+    return p
+
+
+def permute_array(ctx, arr, perm):
+    """TO-permuted copy of `arr` (tile data, tiled range and shape)."""
+    perm = [int(x) for x in perm]
+    rank = arr.trange.rank
+    new_tr = arr.trange.permuted(perm)
+    ext = arr.trange.tile_extents_table()
+    offs = arr.offsets
+    # tile ordinal of every source tile in the permuted grid
+    grid = np.arange(arr.trange.ntiles).reshape(arr.trange.tiles_shape)
+    inv = np.argsort(perm)
+    new_ord_of_src = np.empty(arr.trange.ntiles, np.int64)
+    new_ord_of_src[np.transpose(grid, inv).reshape(-1)] = np.arange(arr.trange.ntiles)
+    new_offs = np.full(arr.trange.ntiles, -1, np.int64)
+    new_offs[new_ord_of_src] = offs  # each tile keeps its place in the pool
+    present = offs >= 0
+    plan = PermutePlan(ctx, perm, ext[present], offs[present], offs[present])
+    pool = torch.empty_like(arr.pool)
+    if plan.volume:
+        plan.run(arr.pool, pool)
+    out = DistArray(new_tr, pool, new_offs)
+    if arr.shape_norms is not None:
+        out.shape_norms = arr.shape_norms.view(arr.trange.tiles_shape).permute(
+            *[int(i) for i in inv]).contiguous().view(-1)
+        out.shape_threshold = arr.shape_threshold
+    out._perm_plan = plan
+    return out
+
+
+# -------------------------------------------------------- contraction ------
+
+class Contraction:
+    """A prepared single-rank contraction C(target) = alpha * A(left) * B(right).
+
+    Host work (planning, operand permutation plans, fusion metadata, result
+    allocation) happens here; run() enqueues device work only.
+    """
+
+    def __init__(self, ctx, target, A, left, B, right, alpha=1.0, threshold=FLOAT_EPS,
+                 pair_threshold=0.0, precision="f64", tile=None, permute_operands=True):
+        self.ctx = ctx
+        self.alpha = float(alpha)
+        self.precision = precision
+        self.tile = tile
+        self.plan = plan = plan_contraction(left, right, target)
+        self.threshold = float(np.float32(threshold))
+        dev = A.pool.device
+        # operands in GEMM layout
+        self.A_src, self.B_src = A, B
+        self.A = permute_array(ctx, A, plan.left_perm[:plan.left_rank]) \
+            if plan.left_permtype == 3 else A
+        self.B = permute_array(ctx, B, plan.right_perm[:plan.right_rank]) \
+            if plan.right_permtype == 3 else B
+        self.permute_operands = permute_operands
+        Ag, Bg = self.A, self.B
+        l_inner = [bool(plan.left_gemm_is_inner[d]) for d in range(plan.left_rank)]
+        r_inner = [bool(plan.right_gemm_is_inner[d]) for d in range(plan.right_rank)]
+        l_outer_r = [Ag.trange.ranges[d] for d in range(plan.left_rank) if not l_inner[d]]
+        l_inner_r = [Ag.trange.ranges[d] for d in range(plan.left_rank) if l_inner[d]]
+        r_outer_r = [Bg.trange.ranges[d] for d in range(plan.right_rank) if not r_inner[d]]
+        r_inner_r = [Bg.trange.ranges[d] for d in range(plan.right_rank) if r_inner[d]]
+        # contracted modes must be tiled identically (cont_engine.h:614-633)
+        if len(l_inner_r) != len(r_inner_r) or any(a != b for a, b in zip(l_inner_r, r_inner_r)):
+            raise ValueError("contracted modes of the operands have incongruent tilings")
+        self.op_a, self.op_b = int(plan.op_a), int(plan.op_b)
+        self.Mt = int(np.prod([r.ntiles for r in l_outer_r], dtype=np.int64))
+        self.Nt = int(np.prod([r.ntiles for r in r_outer_r], dtype=np.int64))
+        self.Kt = int(np.prod([r.ntiles for r in l_inner_r], dtype=np.int64))
+        mext, next_, kext = (_fused_extents(l_outer_r), _fused_extents(r_outer_r),
+                             _fused_extents(l_inner_r))
+        self.mext_h, self.next_h, self.kext_h = mext, next_, kext
+        self.d_mext = torch.as_tensor(mext.astype(np.int32), device=dev)
+        self.d_next = torch.as_tensor(next_.astype(np.int32), device=dev)
+        self.d_kext = torch.as_tensor(kext.astype(np.int32), device=dev)
+        self.k_sizes = torch.as_tensor(k_sizes(l_inner_r), device=dev)
+        self.sparse = Ag.shape_norms is not None and Bg.shape_norms is not None
+        if (Ag.shape_norms is None) != (Bg.shape_norms is None):
+            raise ValueError("operands must both be dense or both be sparse")
+        # result structure in natural order (left outer, right outer)
+        self.natural_ranges = l_outer_r + r_outer_r
+        self.c_norms = torch.ones(self.Mt * self.Nt, dtype=torch.float32, device=dev)
+        self.c_off = torch.empty(self.Mt * self.Nt, dtype=torch.int64, device=dev)
+        self.c_capacity = int(mext.sum() * next_.sum())
+        self.dtype = A.pool.dtype
+        self.C = torch.empty(self.c_capacity, dtype=self.dtype, device=dev)
+        self.store_trans = bool(plan.result_perm_matrix_transpose)
+        grid = self.Mt * self.Nt
+        self.wl = WorkList(ctx, grid, max(1, grid * self.Kt), grid)
+        self.pair_threshold = float(pair_threshold)
+        self._desc = self._make_desc()
+
+    def _make_desc(self):
+        d = _lib.ContractionDesc()
+        d.op_a, d.op_b = self.op_a, self.op_b
+        d.Mt, d.Nt, d.Kt, d.nbatch = self.Mt, self.Nt, self.Kt, 1
+        d.mext, d.next, d.kext = (self.d_mext.data_ptr(), self.d_next.data_ptr(),
+                                  self.d_kext.data_ptr())
+        d.a_off, d.b_off = self.A.d_offsets.data_ptr(), self.B.d_offsets.data_ptr()
+        if self.sparse:
+            d.a_norms, d.b_norms = self.A.shape_norms.data_ptr(), self.B.shape_norms.data_ptr()
+            d.a_threshold, d.b_threshold = self.A.shape_threshold, self.B.shape_threshold
+            d.c_threshold = self.threshold
+        else:
+            d.a_norms = d.b_norms = None
+            d.a_threshold = d.b_threshold = 0.0
+            d.c_threshold = 0.0
+        d.c_norms = self.c_norms.data_ptr()
+        d.pair_threshold = self.pair_threshold
+        d.store_trans = 1 if self.store_trans else 0
+        d.c_off = self.c_off.data_ptr()
+        d.c_capacity = self.c_capacity
+        return d
+
+    # -- device steps -------------------------------------------------------
+    def permute_step(self):
+        """Re-permutes `general` operands from their source arrays."""
+        for src, dst in ((self.A_src, self.A), (self.B_src, self.B)):
+            if dst is not src:
+                dst._perm_plan.run(src.pool, dst.pool)
+
+    def shape_step(self):
+        """Result shape estimate (SparseShape::gemm) into c_norms."""
+        if not self.sparse:
+            return
+        call("tamx_shape_gemm_f32", self.ctx.ptr, self.A.shape_norms.data_ptr(),
+             self.B.shape_norms.data_ptr(), self.op_a, self.op_b, self.Mt, self.Nt, self.Kt, 1,
+             self.k_sizes.data_ptr(), ctypes.c_float(abs(self.alpha)),
+             ctypes.c_float(self.threshold), self.c_norms.data_ptr(), None, self.ctx.stream())
+
+    def worklist_step(self):
+        call("tamx_contraction_worklist", self.ctx.ptr, ctypes.byref(self._desc), self.wl.ptr,
+             self.ctx.stream())
+
+    def gemm_step(self):
+        s = self.ctx.stream()
+        A, B, C = self.A.pool.data_ptr(), self.B.pool.data_ptr(), self.C.data_ptr()
+        if self.precision == "f64":
+            if self.tile is None:
+                call("tamx_grouped_gemm_f64", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
+                     ctypes.c_double(self.alpha), A, B, C, s)
+            else:
+                call("tamx_grouped_gemm_f64_tiled", self.ctx.ptr, self.wl.ptr, self.op_a,
+                     self.op_b, ctypes.c_double(self.alpha), A, B, C, int(self.tile[0]),
+                     int(self.tile[1]), s)
+        elif self.precision == "f32":
+            call("tamx_grouped_gemm_f32", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
+                 ctypes.c_float(self.alpha), A, B, C, s)
+        else:
+            raise ValueError(f"unknown precision {self.precision!r}")
+
+    def run(self):
+        if self.permute_operands:
+            self.permute_step()
+        self.shape_step()
+        self.worklist_step()
+        self.gemm_step()
+
+    # -- results ------------------------------------------------------------
+    def counters(self):
+        c = self.wl.counters()
+        if c[_lib.WL_STATUS] != 0:
+            raise _lib.TamxError(c[_lib.WL_STATUS], "tamx_contraction_worklist",
+                                 f"capacity exceeded (tasks {c[5]}, pairs {c[6]})")
+        return {"ntasks": c[_lib.WL_NTASKS], "npairs": c[_lib.WL_NPAIRS],
+                "celems": c[_lib.WL_CELEMS], "flops": c[_lib.WL_FLOPS]}
+
+    def result(self):
+        """The result DistArray in target order (synchronises)."""
+        plan = self.plan
+        nat = TiledRange(self.natural_ranges)
+        rp = [int(plan.result_perm[d]) for d in range(plan.result_rank)]
+        norms = self.c_norms if self.sparse else None
+        thr = self.threshold if self.sparse else None
+        if plan.result_perm_identity:
+            return DistArray(nat, self.C, d_offsets=self.c_off, shape_norms=norms,
+                             shape_threshold=thr)
+        tgt = nat.permuted(rp)
+        inv = [int(i) for i in np.argsort(rp)]
+        grid_perm = lambda t: t.view(nat.tiles_shape).permute(*inv).contiguous().view(-1)
+        tnorms = grid_perm(norms) if norms is not None else None
+        if self.store_trans:
+            return DistArray(tgt, self.C, d_offsets=grid_perm(self.c_off), shape_norms=tnorms,
+                             shape_threshold=thr)
+        # general result permutation: permute every non-zero tile in place
+        # of its pool slot (contract_reduce.h:370-378)
+        offs = self.c_off.cpu().numpy()
+        present = offs >= 0
+        ext = nat.tile_extents_table()
+        pplan = PermutePlan(self.ctx, rp, ext[present], offs[present], offs[present])
+        out = torch.empty_like(self.C)
+        if pplan.volume:
+            pplan.run(self.C, out)
+        return DistArray(tgt, out, d_offsets=grid_perm(self.c_off), shape_norms=tnorms,
+                         shape_threshold=thr)
+
+
+def contract(ctx, target, A, left, B, right, **kw):
+    """C(target) = alpha * A(left) * B(right); returns (result, counters)."""
+    c = Contraction(ctx, target, A, left, B, right, **kw)
+    c.run()
+    return c.result(), c.counters()
