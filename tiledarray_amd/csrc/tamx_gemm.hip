@@ -11,8 +11,9 @@
 // on C.
 //
 // MFMA fragment layout of v_mfma_{f64,f32}_16x16x4: lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16]; the 16x16 result is held as
-// D[4 * (l / 16) + v][l % 16] in accumulator component v.
+// A[l % 16][l / 16] and B[l / 16][l % 16].  The 16x16 result is held in
+// column l % 16; accumulator component v is row 4 * (l / 16) + v for f32 and
+// row (l / 16) + 4 * v for f64.
 #include <algorithm>
 #include <mutex>
 #include <unordered_map>
@@ -33,6 +34,7 @@ struct Mfma<double> {
   __device__ static inline acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int v) { return lk + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -40,6 +42,7 @@ struct Mfma<float> {
   __device__ static inline acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int v) { return 4 * lk + v; }
 };
 
 template <typename T>
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = wr + i * 16 + lk * 4 + v;
+          const int row = wr + i * 16 + M::row(lk, v);
           const int col = wc + j * 16 + lrow;
           if (row < mrem && col < nrem) {
             const int64_t idx = trans ? (int64_t)(c0 + col) * task.ldc + r0 + row

@@ -2,10 +2,11 @@
 // the float norm-GEMM result estimate, and the work-list builder that turns
 // the screened shapes into the tasks/pairs of a grouped GEMM.
 //
-// Float arithmetic that decides a sparsity pattern is written with explicit
-// round-to-nearest intrinsics (__fmul_rn, __fadd_rn, __fdiv_rn) so no
-// multiply-add is ever contracted and every step rounds exactly where the
-// reference's expression does; the pattern is then reproducible bit for bit.
+// Float arithmetic that decides a sparsity pattern is compiled with FP
+// contraction off (#pragma clang fp contract(off) in each such kernel; the
+// __fmul_rn/__fadd_rn spellings alone do not stop clang from fusing a
+// multiply and an add), so every step rounds exactly where the reference's
+// expression does and the pattern is reproducible bit for bit.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -121,6 +122,7 @@ struct ScaleParams {
 // :177-181) evaluated for one multi-index: modes [lo, hi).
 __device__ float inv_rop(const ScaleParams& p, const float* sizes,
                          const int64_t* idx, int lo, int hi) {
+#pragma clang fp contract(off)
   if (hi - lo == 1) return __fdiv_rn(1.0f, sizes[p.mode_off[lo] + idx[lo]]);
   const int n = hi - lo;
   const int middle = (n >> 1) + (n & 1);
@@ -130,6 +132,7 @@ __device__ float inv_rop(const ScaleParams& p, const float* sizes,
 }
 
 __global__ void shape_scale_kernel(ScaleParams p) {
+#pragma clang fp contract(off)
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.per_batch * p.nbatch) return;
   const int64_t b = g / p.per_batch;
@@ -174,6 +177,7 @@ struct ShapeGemmParams {
 // (sparse_shape.h:1632-1647): left[s] *= k_sizes[s % K], right[s] *=
 // k_sizes[s / N].  One thread per result tile, k ascending.
 __global__ void shape_gemm_kernel(ShapeGemmParams p) {
+#pragma clang fp contract(off)
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.M * p.N * p.nbatch) return;
   const int64_t b = g / (p.M * p.N);
@@ -227,6 +231,7 @@ __device__ inline bool b_present(const tamx_contraction_desc& d, int64_t bi) {
 }
 __device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
                                  int64_t bi, float thr_k) {
+#pragma clang fp contract(off)
   if (!a_present(d, ai) || !b_present(d, bi)) return false;
   if (thr_k > 0.0f && d.a_norms && d.b_norms &&
       __fmul_rn(d.a_norms[ai], d.b_norms[bi]) < thr_k)
@@ -237,6 +242,7 @@ __device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
 // Pass 1: per result tile, the number of contributing pairs, the element
 // count and the flops (zero for result tiles screened out by the shape).
 __global__ void wl_count_kernel(WlParams p) {
+#pragma clang fp contract(off)
   const tamx_contraction_desc& d = p.d;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = d.Mt * d.Nt;
@@ -301,6 +307,7 @@ __global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p) {
 
 // Pass 3: write tasks, pairs (k ascending) and the C offsets.
 __global__ void wl_fill_kernel(WlParams p) {
+#pragma clang fp contract(off)
   const tamx_contraction_desc& d = p.d;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = d.Mt * d.Nt;
