@@ -34,7 +34,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="blocksparse", choices=["blocksparse", "dense"])
     p.add_argument("--n", type=int, default=None, help="matrix extent (default per workload)")
-    p.add_argument("--tile", default=None, help="macro tile BMxBN for the grouped GEMM")
+    p.add_argument("--tile", default=None, help="macro tile BMxBN[xBK[xWGS]] for the grouped GEMM")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     return p.parse_args()
@@ -131,7 +131,7 @@ def main():
             "result_tiles": cnt["ntasks"],
             "tile_pairs": cnt["npairs"],
             "flops_per_step": flops,
-            "macro_tile": list(tile) if tile else [64, 64],
+            "macro_tile": list(tile) if tile else [64, 64, 16],
             "seed": info["seed"],
         },
         "roofline": {

@@ -283,12 +283,24 @@ int tamx_permute_f32(tamx_context* ctx, const tamx_permute_plan* plan,
 int tamx_grouped_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, double alpha, const double* A,
                           const double* B, double* C, void* stream);
-/* Same, with the BM x BN macro tile chosen by the caller (an autotuner):
- * (64,64), (128,128), (128,64), (64,128) or (32,32). */
-int tamx_grouped_gemm_f64_tiled(tamx_context* ctx, tamx_worklist* wl,
-                                int op_a, int op_b, double alpha,
-                                const double* A, const double* B, double* C,
-                                int bm, int bn, void* stream);
+/* Launch configuration of a grouped GEMM: a BM x BN macro tile per
+ * workgroup, K staged through LDS in chunks of BK, and at most wgs_per_cu
+ * resident workgroups per compute unit (0 = as many as fit).  The compiled
+ * shapes are enumerated by tamx_gemm_config_count/tamx_gemm_config_get;
+ * the autotuner picks among them. */
+typedef struct tamx_gemm_config {
+  int32_t bm, bn, bk, wgs_per_cu;
+} tamx_gemm_config;
+int tamx_gemm_config_count(void);
+int tamx_gemm_config_get(int index, tamx_gemm_config* cfg);
+int tamx_grouped_gemm_f64_config(tamx_context* ctx, tamx_worklist* wl,
+                                 int op_a, int op_b, double alpha,
+                                 const double* A, const double* B, double* C,
+                                 const tamx_gemm_config* cfg, void* stream);
+int tamx_grouped_gemm_f32_config(tamx_context* ctx, tamx_worklist* wl,
+                                 int op_a, int op_b, float alpha,
+                                 const float* A, const float* B, float* C,
+                                 const tamx_gemm_config* cfg, void* stream);
 /* FP32 data, FP32 MFMA: tensor-of-tensor cells in single precision. */
 int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, float alpha, const float* A,

@@ -62,7 +62,9 @@ def oracle_contract(c, A, B, a_present=None, b_present=None):
 
 
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-@pytest.mark.parametrize("tile", [(64, 64), (128, 128), (32, 32), (128, 64), (64, 128)])
+@pytest.mark.parametrize("tile", [(64, 64, 16), (64, 64, 32), (128, 128, 16), (128, 64, 16),
+                                  (64, 128, 16), (32, 32, 16), (32, 32, 32), (128, 64, 32),
+                                  (64, 128, 32)])
 def test_grouped_gemm_integer_exact_all_ops(ctx, op_a, op_b, tile):
     rng = np.random.default_rng(100 + op_a * 2 + op_b)
     trm = TiledRange1([0, 40, 130, 200, 333])

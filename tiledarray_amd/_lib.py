@@ -49,6 +49,23 @@ class ContractionDesc(ctypes.Structure):
                 ("c_off", ctypes.c_void_p), ("c_capacity", ctypes.c_int64)]
 
 
+class GemmConfig(ctypes.Structure):
+    _fields_ = [("bm", ctypes.c_int32), ("bn", ctypes.c_int32), ("bk", ctypes.c_int32),
+                ("wgs_per_cu", ctypes.c_int32)]
+
+    def key(self):
+        return (self.bm, self.bn, self.bk, self.wgs_per_cu)
+
+
+def gemm_configs():
+    out = []
+    for i in range(lib().tamx_gemm_config_count()):
+        c = GemmConfig()
+        call("tamx_gemm_config_get", i, ctypes.byref(c))
+        out.append(c)
+    return out
+
+
 class ContractionPlan(ctypes.Structure):
     _fields_ = [("left_rank", ctypes.c_int32), ("right_rank", ctypes.c_int32),
                 ("result_rank", ctypes.c_int32), ("inner_rank", ctypes.c_int32),
@@ -100,10 +117,14 @@ SIGNATURES = {
     "tamx_permute_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_permute_f32": [_vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f64": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp],
-    "tamx_grouped_gemm_f64_tiled": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _i, _i, _vp],
+    "tamx_gemm_config_count": [],
+    "tamx_gemm_config_get": [_i, _vp],
+    "tamx_grouped_gemm_f64_config": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_f32_config": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {
+    "tamx_gemm_config_count": ctypes.c_int,
     "tamx_status_string": ctypes.c_char_p,
     "tamx_last_error": ctypes.c_char_p,
     "tamx_version": ctypes.c_char_p,

@@ -26,8 +26,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-function", "-Wno-pass-failed"]
 
 
+# per-file flags: screening arithmetic decides sparsity patterns and must not
+# be contracted into FMAs
+FILE_FLAGS = {"tamx_shape.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src, obj, verbose):
-    cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

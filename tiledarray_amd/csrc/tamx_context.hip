@@ -1,4 +1,5 @@
 // Context, status strings and work-list storage of libtamx.
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -110,6 +111,10 @@ int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
   alloc(&wl->tile_pair_begin, sizeof(int32_t) * gc);
   alloc(&wl->tile_c_begin, sizeof(int64_t) * gc);
   alloc(&wl->item_prefix, sizeof(int32_t) * (tc + 1));
+  // room for 16 macro tiles per task; larger lists fall back to a binary
+  // search over item_prefix inside the GEMM kernel
+  wl->item_capacity = std::min<int64_t>(tc * 16, (int64_t(1) << 31) - 1);
+  alloc(&wl->item_task, sizeof(int32_t) * wl->item_capacity);
   alloc(&wl->queue, sizeof(int32_t) * 4);
   if (e == hipSuccess) e = hipMemset(wl->counters, 0, sizeof(int64_t) * TAMX_WL_NCOUNTERS);
   if (e == hipSuccess) e = hipMemset(wl->queue, 0, sizeof(int32_t) * 4);
@@ -128,7 +133,7 @@ int tamx_worklist_destroy(tamx_worklist* wl) {
   void* ptrs[] = {wl->tasks,       wl->pairs,          wl->counters,
                   wl->tile_npairs, wl->tile_volume,    wl->tile_flops,
                   wl->tile_task,   wl->tile_pair_begin, wl->tile_c_begin,
-                  wl->item_prefix, wl->queue};
+                  wl->item_prefix, wl->item_task, wl->queue};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete wl;

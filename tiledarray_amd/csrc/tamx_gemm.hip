@@ -54,28 +54,40 @@ struct GemmParams {
   const tamx_gemm_pair* __restrict__ pairs;
   const int64_t* __restrict__ counters;
   const int32_t* __restrict__ prefix;
+  const int32_t* __restrict__ item_task;
   int32_t* queue;
   int64_t task_capacity;
+  int64_t item_capacity;
   T alpha;
 };
 
 constexpr int kThreads = 256;
 
-// prefix[t] = number of BM x BN macro tiles of tasks 0..t-1.
+// prefix[t] = number of BM x BN macro tiles of tasks 0..t-1, and the
+// item -> task map when it fits in item_capacity entries.
 __global__ __launch_bounds__(1024) void plan_items_kernel(
     const tamx_gemm_task* __restrict__ tasks, const int64_t* __restrict__ counters,
-    int bm, int bn, int32_t* __restrict__ prefix, int64_t capacity) {
+    int bm, int bn, int32_t* __restrict__ prefix, int32_t* __restrict__ item_task,
+    int64_t capacity, int64_t item_capacity) {
   __shared__ int32_t part[1024];
   int64_t n = counters[TAMX_WL_NTASKS];
   if (n < 0 || n > capacity || counters[TAMX_WL_STATUS] != 0) n = 0;
+  auto count = [&](int64_t i) -> int32_t {
+    const tamx_gemm_task t = tasks[i];
+    return ((t.m + bm - 1) / bm) * ((t.n + bn - 1) / bn);
+  };
   const int32_t total = tamx::block_exclusive_scan<1024, int32_t>(
-      n, part,
-      [&](int64_t i) -> int32_t {
-        const tamx_gemm_task t = tasks[i];
-        return ((t.m + bm - 1) / bm) * ((t.n + bn - 1) / bn);
-      },
-      [&](int64_t i, int32_t v) { prefix[i] = v; });
+      n, part, count, [&](int64_t i, int32_t v) { prefix[i] = v; });
   if (threadIdx.x == 0) prefix[n] = total;
+  if (total <= item_capacity) {
+    // each thread fills the items of a contiguous range of tasks
+    const int64_t seg = (n + 1023) / 1024;
+    const int64_t b = min<int64_t>(n, threadIdx.x * seg), e = min<int64_t>(n, b + seg);
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t q0 = prefix[i], q1 = q0 + count(i);
+      for (int32_t q = q0; q < q1; ++q) item_task[q] = (int32_t)i;
+    }
+  }
 }
 
 template <typename T, int BM, int BN, int BK, bool TA, bool TB>
@@ -111,10 +123,16 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     __syncthreads();
     if (item >= total) break;
 
-    int lo = 0, hi = ntasks - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (p.prefix[mid] <= item) lo = mid; else hi = mid - 1;
+    int lo;
+    if (total <= p.item_capacity) {
+      lo = p.item_task[item];
+    } else {
+      lo = 0;
+      int hi = ntasks - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.prefix[mid] <= item) lo = mid; else hi = mid - 1;
+      }
     }
     const tamx_gemm_task task = p.tasks[lo];
     const int local = item - p.prefix[lo];
@@ -178,18 +196,28 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
         sB[buf][kk][c] = rb[e];
       }
     };
+    // 16x16 blocks that lie wholly outside a ragged edge are skipped (the
+    // branch is uniform across the wave)
+    int vi = 0, vj = 0;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) vi += (wr + i * 16 < mrem) ? 1 : 0;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) vj += (wc + j * 16 < nrem) ? 1 : 0;
     auto compute_chunk = [&](int buf) {
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
         T af[FM], bf[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
+        for (int i = 0; i < FM; ++i)
+          if (i < vi) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
+        for (int j = 0; j < FN; ++j)
+          if (j < vj) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j)
+            if (i < vi && j < vj) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
       }
     };
 
@@ -261,10 +289,9 @@ struct TileCfg {
   int bm, bn;
 };
 
-template <typename T, int BM, int BN, bool TA, bool TB>
+template <typename T, int BM, int BN, int BK, bool TA, bool TB>
 int launch_cfg(tamx_context* ctx, tamx_worklist* wl, const GemmParams<T>& prm,
-               hipStream_t s) {
-  constexpr int BK = 16;
+               hipStream_t s, int wgs_per_cu) {
   auto kern = grouped_gemm_kernel<T, BM, BN, BK, TA, TB>;
   static std::mutex mu;
   static std::unordered_map<int, int> occ_cache;
@@ -281,64 +308,100 @@ int launch_cfg(tamx_context* ctx, tamx_worklist* wl, const GemmParams<T>& prm,
       occ = it->second;
     }
   }
-  const int grid = ctx->num_cus * occ;
+  const int grid = ctx->num_cus * (wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ);
   plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, BM, BN,
-                                       wl->item_prefix, wl->task_capacity);
+                                       wl->item_prefix, wl->item_task,
+                                       wl->task_capacity, wl->item_capacity);
   TAMX_LAUNCH_CHECK();
   kern<<<grid, kThreads, 0, s>>>(prm);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
 }
 
-template <typename T, int BM, int BN>
+template <typename T, int BM, int BN, int BK>
 int launch_ops(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
-               const GemmParams<T>& prm, hipStream_t s) {
-  if (!op_a && !op_b) return launch_cfg<T, BM, BN, false, false>(ctx, wl, prm, s);
-  if (!op_a && op_b) return launch_cfg<T, BM, BN, false, true>(ctx, wl, prm, s);
-  if (op_a && !op_b) return launch_cfg<T, BM, BN, true, false>(ctx, wl, prm, s);
-  return launch_cfg<T, BM, BN, true, true>(ctx, wl, prm, s);
+               const GemmParams<T>& prm, hipStream_t s, int wgs) {
+  if (!op_a && !op_b) return launch_cfg<T, BM, BN, BK, false, false>(ctx, wl, prm, s, wgs);
+  if (!op_a && op_b) return launch_cfg<T, BM, BN, BK, false, true>(ctx, wl, prm, s, wgs);
+  if (op_a && !op_b) return launch_cfg<T, BM, BN, BK, true, false>(ctx, wl, prm, s, wgs);
+  return launch_cfg<T, BM, BN, BK, true, true>(ctx, wl, prm, s, wgs);
 }
+
+// Macro-tile shapes compiled into the library (BM, BN, BK).
+constexpr int kConfigs[][3] = {{64, 64, 16}, {64, 64, 32}, {128, 128, 16},
+                               {128, 64, 16}, {64, 128, 16}, {32, 32, 16},
+                               {32, 32, 32}, {128, 64, 32}, {64, 128, 32}};
 
 template <typename T>
 int grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
-                 T alpha, const T* A, const T* B, T* C, void* stream, int bm,
-                 int bn) {
+                 T alpha, const T* A, const T* B, T* C, void* stream,
+                 const tamx_gemm_config* cfg) {
   TAMX_REQUIRE(wl != nullptr, "null work list");
   TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be TAMX_NO_TRANS or TAMX_TRANS");
   TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be TAMX_NO_TRANS or TAMX_TRANS");
   if (int rc = tamx::activate(ctx)) return rc;
   GemmParams<T> prm{A, B, C, wl->tasks, wl->pairs, wl->counters,
-                    wl->item_prefix, wl->queue, wl->task_capacity, alpha};
+                    wl->item_prefix, wl->item_task, wl->queue,
+                    wl->task_capacity, wl->item_capacity, alpha};
   hipStream_t s = tamx::as_stream(stream);
-  if (bm == 128 && bn == 128) return launch_ops<T, 128, 128>(ctx, wl, op_a, op_b, prm, s);
-  if (bm == 128 && bn == 64) return launch_ops<T, 128, 64>(ctx, wl, op_a, op_b, prm, s);
-  if (bm == 64 && bn == 128) return launch_ops<T, 64, 128>(ctx, wl, op_a, op_b, prm, s);
-  if (bm == 32 && bn == 32) return launch_ops<T, 32, 32>(ctx, wl, op_a, op_b, prm, s);
-  return launch_ops<T, 64, 64>(ctx, wl, op_a, op_b, prm, s);
+  const int bm = cfg ? cfg->bm : 64, bn = cfg ? cfg->bn : 64, bk = cfg ? cfg->bk : 16;
+  const int w = cfg ? cfg->wgs_per_cu : 0;
+#define TAMX_CFG(BM_, BN_, BK_) \
+  if (bm == BM_ && bn == BN_ && bk == BK_) return launch_ops<T, BM_, BN_, BK_>(ctx, wl, op_a, op_b, prm, s, w);
+  TAMX_CFG(64, 64, 16)
+  TAMX_CFG(64, 64, 32)
+  TAMX_CFG(128, 128, 16)
+  TAMX_CFG(128, 64, 16)
+  TAMX_CFG(64, 128, 16)
+  TAMX_CFG(32, 32, 16)
+  TAMX_CFG(32, 32, 32)
+  TAMX_CFG(128, 64, 32)
+  TAMX_CFG(64, 128, 32)
+#undef TAMX_CFG
+  tamx::set_error("unsupported macro tile: see tamx_gemm_config_list");
+  return TAMX_ERR_UNSUPPORTED;
 }
 
 }  // namespace
 
 extern "C" {
 
-// Macro-tile shape override for tuning: 0 selects the default (64 x 64).
-int tamx_grouped_gemm_f64_tiled(tamx_context* ctx, tamx_worklist* wl, int op_a,
-                                int op_b, double alpha, const double* A,
-                                const double* B, double* C, int bm, int bn,
-                                void* stream) {
-  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, bm, bn);
+int tamx_gemm_config_count(void) { return (int)(sizeof(kConfigs) / sizeof(kConfigs[0])); }
+
+int tamx_gemm_config_get(int index, tamx_gemm_config* cfg) {
+  TAMX_REQUIRE(cfg != nullptr, "null config");
+  TAMX_REQUIRE(index >= 0 && index < tamx_gemm_config_count(), "config index out of range");
+  cfg->bm = kConfigs[index][0];
+  cfg->bn = kConfigs[index][1];
+  cfg->bk = kConfigs[index][2];
+  cfg->wgs_per_cu = 0;
+  return TAMX_OK;
+}
+
+int tamx_grouped_gemm_f64_config(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                 int op_b, double alpha, const double* A,
+                                 const double* B, double* C,
+                                 const tamx_gemm_config* cfg, void* stream) {
+  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
+}
+
+int tamx_grouped_gemm_f32_config(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                 int op_b, float alpha, const float* A,
+                                 const float* B, float* C,
+                                 const tamx_gemm_config* cfg, void* stream) {
+  return grouped_gemm<float>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
 }
 
 int tamx_grouped_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, double alpha, const double* A,
                           const double* B, double* C, void* stream) {
-  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, 64, 64);
+  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
 }
 
 int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, float alpha, const float* A,
                           const float* B, float* C, void* stream) {
-  return grouped_gemm<float>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, 64, 64);
+  return grouped_gemm<float>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
 }
 
 }  // extern "C"

@@ -47,6 +47,8 @@ struct tamx_worklist {
   // scratch of the grouped GEMM: macro-tile prefix over tasks and the
   // persistent-kernel queue (next item, finished workgroups)
   int32_t* item_prefix = nullptr;  // task_capacity + 1
+  int32_t* item_task = nullptr;    // item -> task map, item_capacity entries
+  int64_t item_capacity = 0;
   int32_t* queue = nullptr;        // 4 int32, zero between launches
 };
 

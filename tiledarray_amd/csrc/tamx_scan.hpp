@@ -20,7 +20,7 @@ __device__ inline V block_exclusive_scan(int64_t n, V* part, Load load,
   const int64_t seg = (n + NT - 1) / NT;
   const int64_t b = (int64_t)t * seg < n ? (int64_t)t * seg : n;
   const int64_t e = b + seg < n ? b + seg : n;
-  V sum = 0;
+  V sum(0);
   for (int64_t i = b; i < e; ++i) sum += load(i);
   part[t] = sum;
   __syncthreads();

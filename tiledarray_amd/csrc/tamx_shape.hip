@@ -2,11 +2,13 @@
 // the float norm-GEMM result estimate, and the work-list builder that turns
 // the screened shapes into the tasks/pairs of a grouped GEMM.
 //
-// Float arithmetic that decides a sparsity pattern is compiled with FP
-// contraction off (#pragma clang fp contract(off) in each such kernel; the
-// __fmul_rn/__fadd_rn spellings alone do not stop clang from fusing a
-// multiply and an add), so every step rounds exactly where the reference's
-// expression does and the pattern is reproducible bit for bit.
+// Float arithmetic that decides a sparsity pattern is written with plain
+// operators and compiled with FP contraction off (this file is built with
+// -ffp-contract=off, and each such kernel also says so with
+// #pragma clang fp contract(off)), so every product and sum rounds exactly
+// where the reference's expression does and the pattern is reproducible bit
+// for bit.  (HIP's __fmul_rn/__fadd_rn are header functions whose operations
+// keep the contractable flag, so they are not used here.)
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -42,12 +44,27 @@ __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
   __shared__ T red[kNormThreads];
   const int64_t c = blockIdx.x;
   const int64_t b = cbeg[c], e = cend[c];
-  T s = T(0);
-  for (int64_t i = b + threadIdx.x; i < e; i += kNormThreads) {
-    const T x = pool[i];
-    s += x * x;
+  // eight independent loads in flight per thread, eight partial sums
+  constexpr int U = 8;
+  T s[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) s[u] = T(0);
+  int64_t i = b + threadIdx.x;
+  for (; i + (U - 1) * kNormThreads < e; i += U * kNormThreads) {
+    T x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = pool[i + u * kNormThreads];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] += x[u] * x[u];
   }
-  red[threadIdx.x] = s;
+  for (; i < e; i += kNormThreads) {
+    const T x = pool[i];
+    s[0] += x * x;
+  }
+  T t = T(0);
+#pragma unroll
+  for (int u = 0; u < U; ++u) t += s[u];
+  red[threadIdx.x] = t;
   __syncthreads();
   for (int w = kNormThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -123,12 +140,12 @@ struct ScaleParams {
 __device__ float inv_rop(const ScaleParams& p, const float* sizes,
                          const int64_t* idx, int lo, int hi) {
 #pragma clang fp contract(off)
-  if (hi - lo == 1) return __fdiv_rn(1.0f, sizes[p.mode_off[lo] + idx[lo]]);
+  if (hi - lo == 1) return 1.0f / sizes[p.mode_off[lo] + idx[lo]];
   const int n = hi - lo;
   const int middle = (n >> 1) + (n & 1);
   const float l = inv_rop(p, sizes, idx, lo, lo + middle);
   const float r = inv_rop(p, sizes, idx, lo + middle, hi);
-  return __fmul_rn(l, r);
+  return l * r;
 }
 
 __global__ void shape_scale_kernel(ScaleParams p) {
@@ -145,12 +162,12 @@ __global__ void shape_scale_kernel(ScaleParams p) {
   }
   float v = p.norms[g];
   if (p.rank == 1) {
-    v = __fdiv_rn(v, sizes[idx[0]]);  // rank-1 divides (sparse_shape.h:160-173)
+    v = v / sizes[idx[0]];  // rank-1 divides (sparse_shape.h:160-173)
   } else {
     const int middle = (p.rank >> 1) + (p.rank & 1);
     const float x = inv_rop(p, sizes, idx, 0, middle);
     const float y = inv_rop(p, sizes, idx, middle, p.rank);
-    v = __fmul_rn(v, __fmul_rn(x, y));  // norm *= x * y (:205-213)
+    v = v * (x * y);  // norm *= x * y (:205-213)
   }
   if (v < p.threshold) {
     v = 0.0f;
@@ -188,17 +205,18 @@ __global__ void shape_gemm_kernel(ShapeGemmParams p) {
   const float* ks = p.ksz + b * p.K;
   float v;
   if (p.K == 0) {
-    v = __fmul_rn(__fmul_rn(L[i], R[j]), p.abs_factor);  // outer product (:1666-1678)
+    v = (L[i] * R[j]) * p.abs_factor;  // outer product (:1666-1678)
   } else {
     float acc = 0.0f;
     for (int64_t k = 0; k < p.K; ++k) {
       const int64_t sl = p.op_a ? k * p.M + i : i * p.K + k;
       const int64_t sr = p.op_b ? j * p.K + k : k * p.N + j;
-      const float l = __fmul_rn(L[sl], ks[sl % p.K]);
-      const float r = __fmul_rn(R[sr], ks[sr / p.N]);
-      acc = __fadd_rn(acc, __fmul_rn(l, r));
+      const float l = L[sl] * ks[sl % p.K];
+      const float r = R[sr] * ks[sr / p.N];
+      const float prod = l * r;
+      acc = acc + prod;
     }
-    v = __fmul_rn(p.abs_factor, acc);
+    v = p.abs_factor * acc;
   }
   if (v < p.threshold) {
     v = 0.0f;
@@ -234,7 +252,7 @@ __device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
 #pragma clang fp contract(off)
   if (!a_present(d, ai) || !b_present(d, bi)) return false;
   if (thr_k > 0.0f && d.a_norms && d.b_norms &&
-      __fmul_rn(d.a_norms[ai], d.b_norms[bi]) < thr_k)
+      d.a_norms[ai] * d.b_norms[bi] < thr_k)
     return false;
   return true;
 }
@@ -254,7 +272,7 @@ __global__ void wl_count_kernel(WlParams p) {
     const int64_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
     vol = m * n;
     const float thr_k = d.pair_threshold > 0.0f
-                            ? __fdiv_rn(d.pair_threshold, (float)d.Kt)
+                            ? d.pair_threshold / (float)d.Kt
                             : 0.0f;
     for (int64_t k = 0; k < d.Kt; ++k) {
       const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
@@ -270,37 +288,51 @@ __global__ void wl_count_kernel(WlParams p) {
   p.tile_flops[g] = flops;
 }
 
-// Pass 2 (one workgroup): exclusive scans give each non-zero result tile its
-// task index, first pair and place in C (row-major tile order).
+// Pass 2 (one workgroup): one exclusive scan of (task flag, pair count,
+// element count, flops) gives each non-zero result tile its task index,
+// first pair and place in C (row-major tile order).
+struct Quad {
+  int64_t t, p, c, f;
+  __device__ Quad(int v = 0) : t(v), p(v), c(v), f(v) {}
+  __device__ Quad(int64_t t_, int64_t p_, int64_t c_, int64_t f_) : t(t_), p(p_), c(c_), f(f_) {}
+  __device__ Quad& operator+=(const Quad& o) {
+    t += o.t;
+    p += o.p;
+    c += o.c;
+    f += o.f;
+    return *this;
+  }
+};
+
 __global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p) {
-  __shared__ int64_t part64[1024];
+  // raw storage: __shared__ objects cannot have constructors
+  __shared__ __align__(16) unsigned char part_raw[sizeof(Quad) * 1024];
+  Quad* part = reinterpret_cast<Quad*>(part_raw);
   const tamx_contraction_desc& d = p.d;
   const int64_t n = d.Mt * d.Nt * d.nbatch;
-  const int64_t ntasks = tamx::block_exclusive_scan<1024, int64_t>(
-      n, part64, [&](int64_t i) -> int64_t { return p.tile_volume[i] > 0 ? 1 : 0; },
-      [&](int64_t i, int64_t v) { p.tile_task[i] = (int32_t)v; });
-  const int64_t npairs = tamx::block_exclusive_scan<1024, int64_t>(
-      n, part64, [&](int64_t i) -> int64_t { return p.tile_npairs[i]; },
-      [&](int64_t i, int64_t v) { p.tile_pair_begin[i] = (int32_t)v; });
-  const int64_t celems = tamx::block_exclusive_scan<1024, int64_t>(
-      n, part64, [&](int64_t i) -> int64_t { return p.tile_volume[i]; },
-      [&](int64_t i, int64_t v) { p.tile_c_begin[i] = v; });
-  const int64_t flops = tamx::block_exclusive_scan<1024, int64_t>(
-      n, part64, [&](int64_t i) -> int64_t { return p.tile_flops[i]; },
-      [&](int64_t, int64_t) {});
+  const Quad tot = tamx::block_exclusive_scan<1024, Quad>(
+      n, part,
+      [&](int64_t i) -> Quad {
+        const int64_t v = p.tile_volume[i];
+        return Quad(v > 0 ? 1 : 0, p.tile_npairs[i], v, p.tile_flops[i]);
+      },
+      [&](int64_t i, const Quad& v) {
+        p.tile_task[i] = (int32_t)v.t;
+        p.tile_pair_begin[i] = (int32_t)v.p;
+        p.tile_c_begin[i] = v.c;
+      });
   if (threadIdx.x == 0) {
     int64_t status = 0;
-    if (ntasks > p.task_capacity || npairs > p.pair_capacity ||
-        celems > d.c_capacity)
+    if (tot.t > p.task_capacity || tot.p > p.pair_capacity || tot.c > d.c_capacity)
       status = TAMX_ERR_CAPACITY;
-    p.counters[TAMX_WL_NTASKS] = status ? 0 : ntasks;
-    p.counters[TAMX_WL_NPAIRS] = status ? 0 : npairs;
-    p.counters[TAMX_WL_CELEMS] = celems;
+    p.counters[TAMX_WL_NTASKS] = status ? 0 : tot.t;
+    p.counters[TAMX_WL_NPAIRS] = status ? 0 : tot.p;
+    p.counters[TAMX_WL_CELEMS] = tot.c;
     p.counters[TAMX_WL_STATUS] = status;
-    p.counters[TAMX_WL_FLOPS] = flops;
-    // record the totals that did not fit, for the host's error message
-    p.counters[5] = ntasks;
-    p.counters[6] = npairs;
+    p.counters[TAMX_WL_FLOPS] = tot.f;
+    // the totals that were asked for, for the host's capacity message
+    p.counters[5] = tot.t;
+    p.counters[6] = tot.p;
     p.counters[7] = 0;
   }
 }
@@ -333,7 +365,7 @@ __global__ void wl_fill_kernel(WlParams p) {
   task.flags = d.store_trans ? TAMX_TASK_STORE_TRANS : 0;
   p.tasks[t] = task;
   const float thr_k =
-      d.pair_threshold > 0.0f ? __fdiv_rn(d.pair_threshold, (float)d.Kt) : 0.0f;
+      d.pair_threshold > 0.0f ? d.pair_threshold / (float)d.Kt : 0.0f;
   int32_t q = pb;
   for (int64_t k = 0; k < d.Kt; ++k) {
     const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);

@@ -253,7 +253,7 @@ class Contraction:
         self.ctx = ctx
         self.alpha = float(alpha)
         self.precision = precision
-        self.tile = tile
+        self.set_tile(tile)
         self.plan = plan = plan_contraction(left, right, target)
         self.threshold = float(np.float32(threshold))
         dev = A.pool.device
@@ -323,6 +323,13 @@ class Contraction:
         d.c_capacity = self.c_capacity
         return d
 
+    def set_tile(self, tile):
+        """tile: None (library default) or (bm, bn[, bk[, wgs_per_cu]])."""
+        self.tile = None if tile is None else tuple(int(x) for x in tile)
+        if self.tile is not None:
+            t = self.tile + (16, 0)[len(self.tile) - 2:] if len(self.tile) < 4 else self.tile
+            self.tile_cfg = _lib.GemmConfig(*t[:4])
+
     # -- device steps -------------------------------------------------------
     def permute_step(self):
         """Re-permutes `general` operands from their source arrays."""
@@ -346,17 +353,13 @@ class Contraction:
     def gemm_step(self):
         s = self.ctx.stream()
         A, B, C = self.A.pool.data_ptr(), self.B.pool.data_ptr(), self.C.data_ptr()
+        cfg = None if self.tile is None else ctypes.byref(self.tile_cfg)
         if self.precision == "f64":
-            if self.tile is None:
-                call("tamx_grouped_gemm_f64", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
-                     ctypes.c_double(self.alpha), A, B, C, s)
-            else:
-                call("tamx_grouped_gemm_f64_tiled", self.ctx.ptr, self.wl.ptr, self.op_a,
-                     self.op_b, ctypes.c_double(self.alpha), A, B, C, int(self.tile[0]),
-                     int(self.tile[1]), s)
+            call("tamx_grouped_gemm_f64_config", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
+                 ctypes.c_double(self.alpha), A, B, C, cfg, s)
         elif self.precision == "f32":
-            call("tamx_grouped_gemm_f32", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
-                 ctypes.c_float(self.alpha), A, B, C, s)
+            call("tamx_grouped_gemm_f32_config", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
+                 ctypes.c_float(self.alpha), A, B, C, cfg, s)
         else:
             raise ValueError(f"unknown precision {self.precision!r}")
 
