@@ -27,8 +27,11 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 
 # per-file flags: screening arithmetic decides sparsity patterns and must not
-# be contracted into FMAs
-FILE_FLAGS = {"tamx_shape.hip": ["-ffp-contract=off"]}
+# be contracted into FMAs; MFMA kernels keep their accumulators in VGPRs
+# (gfx950 MFMAs read and write VGPRs directly), which avoids AGPR<->VGPR
+# copies around every MFMA chain
+FILE_FLAGS = {"tamx_shape.hip": ["-ffp-contract=off"],
+              "tamx_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src, obj, verbose):

@@ -115,9 +115,9 @@ int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
   // search over item_prefix inside the GEMM kernel
   wl->item_capacity = std::min<int64_t>(tc * 16, (int64_t(1) << 31) - 1);
   alloc(&wl->item_task, sizeof(int32_t) * wl->item_capacity);
-  alloc(&wl->queue, sizeof(int32_t) * 4);
+  alloc(&wl->queue, sizeof(int32_t) * 16);
   if (e == hipSuccess) e = hipMemset(wl->counters, 0, sizeof(int64_t) * TAMX_WL_NCOUNTERS);
-  if (e == hipSuccess) e = hipMemset(wl->queue, 0, sizeof(int32_t) * 4);
+  if (e == hipSuccess) e = hipMemset(wl->queue, 0, sizeof(int32_t) * 16);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     tamx::set_error(std::string("worklist allocation: ") + hipGetErrorString(e));

@@ -62,6 +62,8 @@ struct GemmParams {
 };
 
 constexpr int kThreads = 256;
+constexpr int kRegions = 8;
+constexpr int kPairWin = 64;  // pair descriptors staged in LDS at a time  // XCDs of an MI355X; the queue holds kRegions + 1 counters
 
 // prefix[t] = number of BM x BN macro tiles of tasks 0..t-1, and the
 // item -> task map when it fits in item_capacity entries.
@@ -90,6 +92,13 @@ __global__ __launch_bounds__(1024) void plan_items_kernel(
   }
 }
 
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline int64_t uni64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <typename T, int BM, int BN, int BK, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p) {
   using M = Mfma<T>;
@@ -99,46 +108,86 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
   constexpr int SA = BM + 1, SB = BN + 1;  // padded LDS rows
   static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0, "tile shape");
   static_assert((BM * BK) % kThreads == 0 && (BK * BN) % kThreads == 0, "load split");
+  // per-thread coordinates of the elements it stages (fixed for the kernel)
+  // A NoTrans: k fastest (rows of op(A) contiguous along k); A Trans: m fastest
+  constexpr int A_STEP = TA ? kThreads / BM : kThreads / BK;  // row (or k) step per e
+  constexpr int B_STEP = TB ? kThreads / BK : kThreads / BN;
+  static_assert(TA ? kThreads % BM == 0 : kThreads % BK == 0, "A staging layout");
+  static_assert(TB ? kThreads % BK == 0 : kThreads % BN == 0, "B staging layout");
 
   __shared__ T sA[2][BK][SA];
   __shared__ T sB[2][BK][SB];
+  __shared__ tamx_gemm_pair s_pairs[kPairWin];
   __shared__ int s_item;
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = uni(tid >> 6);
   const int wr = (wave >> 1) * (BM / 2), wc = (wave & 1) * (BN / 2);
   const int lrow = lane & 15, lk = lane >> 4;
+  // A staging: element e is (row a_r + e*A_STEP or a_r, k a_k or a_k + e*A_STEP)
+  const int a_fast = TA ? tid % BM : tid % BK;  // m (Trans) or k (NoTrans)
+  const int a_slow = TA ? tid / BM : tid / BK;  // k (Trans) or m (NoTrans)
+  const int b_fast = TB ? tid % BK : tid % BN;  // k (Trans) or n (NoTrans)
+  const int b_slow = TB ? tid / BK : tid / BN;  // n (Trans) or k (NoTrans)
 
   int64_t ntasks64 = p.counters[TAMX_WL_NTASKS];
   // same validation as the planner: an invalid list has no items
   if (ntasks64 < 0 || ntasks64 > p.task_capacity || p.counters[TAMX_WL_STATUS] != 0)
     ntasks64 = 0;
-  const int32_t ntasks = (int32_t)ntasks64;
-  const int32_t total = p.prefix[ntasks > 0 ? ntasks : 0];
+  const int32_t ntasks = uni((int32_t)ntasks64);
+  const int32_t total = uni(p.prefix[ntasks > 0 ? ntasks : 0]);
 
+  // Items are split into kRegions contiguous ranges, one per XCD: a
+  // workgroup drains the range of its own XCD (workgroups are dispatched to
+  // XCDs round-robin by blockIdx), so the macro tiles of one result tile,
+  // which share A and B panels, run on one L2; then it steals from the
+  // other ranges in turn.
+  const int home = blockIdx.x % kRegions;
+  int tried = 0;  // used by thread 0 only
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(&p.queue[0], 1);
+    if (tid == 0) {
+      int it = -1;
+      while (tried < kRegions) {
+        const int r = (home + tried) % kRegions;
+        const int rb = (int)((int64_t)total * r / kRegions);
+        const int re = (int)((int64_t)total * (r + 1) / kRegions);
+        const int q = rb < re ? atomicAdd(&p.queue[r], 1) : 0;
+        if (rb + q < re) {
+          it = rb + q;
+          break;
+        }
+        ++tried;
+      }
+      s_item = it;
+    }
     __syncthreads();
-    const int item = s_item;
+    const int item = uni(s_item);
     __syncthreads();
-    if (item >= total) break;
+    if (item < 0) break;
 
     int lo;
     if (total <= p.item_capacity) {
-      lo = p.item_task[item];
+      lo = uni(p.item_task[item]);
     } else {
       lo = 0;
       int hi = ntasks - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (p.prefix[mid] <= item) lo = mid; else hi = mid - 1;
+        if (uni(p.prefix[mid]) <= item) lo = mid; else hi = mid - 1;
       }
     }
-    const tamx_gemm_task task = p.tasks[lo];
-    const int local = item - p.prefix[lo];
-    const int tiles_n = (task.n + BN - 1) / BN;
+    const tamx_gemm_task* tp = p.tasks + lo;
+    const int64_t t_coff = uni64(tp->c_off);
+    const int t_m = uni(tp->m), t_n = uni(tp->n), t_ldc = uni(tp->ldc);
+    const int t_pb = uni(tp->pair_begin), t_pc = uni(tp->pair_count);
+    const int t_flags = uni(tp->flags);
+    const int local = item - uni(p.prefix[lo]);
+    const int tiles_n = (t_n + BN - 1) / BN;
     const int r0 = (local / tiles_n) * BM, c0 = (local % tiles_n) * BN;
-    const int mrem = min(BM, task.m - r0), nrem = min(BN, task.n - c0);
+    const int mrem = min(BM, t_m - r0), nrem = min(BN, t_n - c0);
+    // a wave whose quadrant lies wholly outside a ragged edge skips its
+    // MFMAs (the condition is uniform across the wave)
+    const bool wave_active = (wr < mrem) && (wc < nrem);
 
     acc_t acc[FM][FN];
 #pragma unroll
@@ -148,115 +197,151 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
 #pragma unroll
         for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
 
-    T ra[A_LD], rb[B_LD];
-    auto load_chunk = [&](const tamx_gemm_pair& pr, int k0) {
-      const int kc = min(BK, pr.k - k0);
-      const T* __restrict__ a = p.A + pr.a_off;
-      const T* __restrict__ b = p.B + pr.b_off;
+    // Per-pair staging state of the load cursor: the thread's element
+    // offsets (relative to the pair's tile) at k0 = 0, and whether its
+    // row/column is inside the tile.
+    int64_t a_base = 0, b_base = 0;
+    int a_lda = 0, b_ldb = 0, pk = 0;
+    int64_t a_off[A_LD], b_off[B_LD];
+    bool a_ok[A_LD], b_ok[B_LD];
+    // pair descriptors of the task: in LDS when they fit, else read from
+    // global memory
+    const int pend = t_pb + t_pc;
+    const bool pairs_in_lds = t_pc <= kPairWin;
+    if (pairs_in_lds && t_pc > 0) {
+      const int32_t* src = reinterpret_cast<const int32_t*>(p.pairs + t_pb);
+      int32_t* dst = reinterpret_cast<int32_t*>(s_pairs);
+      constexpr int W = sizeof(tamx_gemm_pair) / sizeof(int32_t);
+      for (int x = tid; x < t_pc * W; x += kThreads) dst[x] = src[x];
+    }
+    __syncthreads();
+    auto set_pair = [&](int pi) {
+      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
+      a_base = uni64(pp.a_off);
+      b_base = uni64(pp.b_off);
+      pk = uni(pp.k);
+      a_lda = uni(pp.lda);
+      b_ldb = uni(pp.ldb);
 #pragma unroll
       for (int e = 0; e < A_LD; ++e) {
-        const int idx = tid + e * kThreads;
-        int r, kk;
-        if (!TA) { kk = idx % BK; r = idx / BK; }
-        else { r = idx % BM; kk = idx / BM; }
-        T v = T(0);
-        if (r < mrem && kk < kc)
-          v = TA ? a[(int64_t)(k0 + kk) * pr.lda + r0 + r]
-                 : a[(int64_t)(r0 + r) * pr.lda + k0 + kk];
-        ra[e] = v;
+        const int slow = a_slow + e * A_STEP;
+        if (TA) {  // (k = slow, m = a_fast)
+          a_ok[e] = a_fast < mrem;
+          a_off[e] = (int64_t)slow * a_lda + r0 + a_fast;
+        } else {  // (m = slow, k = a_fast)
+          a_ok[e] = slow < mrem;
+          a_off[e] = (int64_t)(r0 + slow) * a_lda + a_fast;
+        }
       }
 #pragma unroll
       for (int e = 0; e < B_LD; ++e) {
-        const int idx = tid + e * kThreads;
-        int c, kk;
-        if (!TB) { c = idx % BN; kk = idx / BN; }
-        else { kk = idx % BK; c = idx / BK; }
-        T v = T(0);
-        if (c < nrem && kk < kc)
-          v = TB ? b[(int64_t)(c0 + c) * pr.ldb + k0 + kk]
-                 : b[(int64_t)(k0 + kk) * pr.ldb + c0 + c];
-        rb[e] = v;
+        const int slow = b_slow + e * B_STEP;
+        if (TB) {  // (n = slow, k = b_fast)
+          b_ok[e] = slow < nrem;
+          b_off[e] = (int64_t)(c0 + slow) * b_ldb + b_fast;
+        } else {  // (k = slow, n = b_fast)
+          b_ok[e] = b_fast < nrem;
+          b_off[e] = (int64_t)slow * b_ldb + c0 + b_fast;
+        }
       }
     };
-    auto store_chunk = [&](int buf) {
+
+    // load cursor (pair lp, offset lk0) runs two chunks ahead of compute
+    int lp = t_pb, lk0 = 0;
+    if (lp < pend) set_pair(lp);
+    auto advance = [&]() {
+      lk0 += BK;
+      if (lk0 >= pk) {
+        ++lp;
+        lk0 = 0;
+        if (lp < pend) set_pair(lp);
+      }
+    };
+    // loads of chunk lk0 of the cursor's pair: out-of-tile elements read
+    // the tile's first element and are replaced by zero (no branches)
+    auto load_chunk = [&](T (&ra)[A_LD], T (&rb)[B_LD]) {
+      const int kc = min(BK, pk - lk0);
+      const T* __restrict__ a = p.A + a_base;
+      const T* __restrict__ b = p.B + b_base;
+      const int64_t a_k = TA ? (int64_t)lk0 * a_lda : lk0;
+      const int64_t b_k = TB ? lk0 : (int64_t)lk0 * b_ldb;
 #pragma unroll
       for (int e = 0; e < A_LD; ++e) {
-        const int idx = tid + e * kThreads;
-        int r, kk;
-        if (!TA) { kk = idx % BK; r = idx / BK; }
-        else { r = idx % BM; kk = idx / BM; }
-        sA[buf][kk][r] = ra[e];
+        const int kk = TA ? a_slow + e * A_STEP : a_fast;
+        const bool ok = a_ok[e] && kk < kc;
+        const T v = a[ok ? a_off[e] + a_k : 0];
+        ra[e] = ok ? v : T(0);
       }
 #pragma unroll
       for (int e = 0; e < B_LD; ++e) {
-        const int idx = tid + e * kThreads;
-        int c, kk;
-        if (!TB) { c = idx % BN; kk = idx / BN; }
-        else { kk = idx % BK; c = idx / BK; }
-        sB[buf][kk][c] = rb[e];
+        const int kk = TB ? b_fast : b_slow + e * B_STEP;
+        const bool ok = b_ok[e] && kk < kc;
+        const T v = b[ok ? b_off[e] + b_k : 0];
+        rb[e] = ok ? v : T(0);
+      }
+      advance();
+    };
+    auto store_chunk = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD]) {
+#pragma unroll
+      for (int e = 0; e < A_LD; ++e) {
+        if (TA) sA[buf][a_slow + e * A_STEP][a_fast] = ra[e];
+        else sA[buf][a_fast][a_slow + e * A_STEP] = ra[e];
+      }
+#pragma unroll
+      for (int e = 0; e < B_LD; ++e) {
+        if (TB) sB[buf][b_fast][b_slow + e * B_STEP] = rb[e];
+        else sB[buf][b_slow + e * B_STEP][b_fast] = rb[e];
       }
     };
-    // 16x16 blocks that lie wholly outside a ragged edge are skipped (the
-    // branch is uniform across the wave)
-    int vi = 0, vj = 0;
-#pragma unroll
-    for (int i = 0; i < FM; ++i) vi += (wr + i * 16 < mrem) ? 1 : 0;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) vj += (wc + j * 16 < nrem) ? 1 : 0;
     auto compute_chunk = [&](int buf) {
+      if (!wave_active) return;
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
         T af[FM], bf[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
-          if (i < vi) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
+        for (int i = 0; i < FM; ++i) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-          if (j < vj) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
+        for (int j = 0; j < FN; ++j) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
-            if (i < vi && j < vj) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
       }
     };
 
-    int pi = task.pair_begin;
-    const int pend = task.pair_begin + task.pair_count;
-    if (pi < pend) {
-      tamx_gemm_pair cur = p.pairs[pi];
-      int k0 = 0;
-      load_chunk(cur, k0);
-      store_chunk(0);
+    // Pipeline: registers R0/R1 hold chunks c+1 and c+2 in flight while
+    // chunk c is multiplied out of LDS buffer c % 2.  One barrier per chunk.
+    T ra0[A_LD], rb0[B_LD], ra1[A_LD], rb1[B_LD];
+    if (lp < pend) {
+      load_chunk(ra0, rb0);                    // chunk 0
+      const bool have1 = lp < pend;
+      if (have1) load_chunk(ra1, rb1);         // chunk 1
+      store_chunk(0, ra0, rb0);
       __syncthreads();
-      int buf = 0;
-      // advance to the chunk after (pi, k0)
-      k0 += BK;
-      if (k0 >= cur.k) {
-        ++pi;
-        k0 = 0;
-        if (pi < pend) cur = p.pairs[pi];
-      }
+      bool more = have1;  // chunk c + 1 exists
       for (;;) {
-        const bool has_next = pi < pend;
-        if (has_next) load_chunk(cur, k0);
-        compute_chunk(buf);
-        if (!has_next) break;
-        store_chunk(buf ^ 1);
+        // even step: compute buffer 0, chunk c+1 is in R1
+        bool next2 = lp < pend;
+        if (next2) load_chunk(ra0, rb0);       // chunk c + 2 -> R0
+        compute_chunk(0);
+        if (!more) break;
+        store_chunk(1, ra1, rb1);
         __syncthreads();
-        buf ^= 1;
-        k0 += BK;
-        if (k0 >= cur.k) {
-          ++pi;
-          k0 = 0;
-          if (pi < pend) cur = p.pairs[pi];
-        }
+        more = next2;
+        // odd step: compute buffer 1, chunk c+1 is in R0
+        next2 = lp < pend;
+        if (next2) load_chunk(ra1, rb1);       // chunk c + 2 -> R1
+        compute_chunk(1);
+        if (!more) break;
+        store_chunk(0, ra0, rb0);
+        __syncthreads();
+        more = next2;
       }
     }
 
-    const bool beta1 = (task.flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (task.flags & TAMX_TASK_STORE_TRANS) != 0;
-    T* __restrict__ c = p.C + task.c_off;
+    const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
+    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
+    T* __restrict__ c = p.C + t_coff;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -266,8 +351,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
           const int row = wr + i * 16 + M::row(lk, v);
           const int col = wc + j * 16 + lrow;
           if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * task.ldc + r0 + row
-                                      : (int64_t)(r0 + row) * task.ldc + c0 + col;
+            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
+                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
             T val = p.alpha * acc[i][j][v];
             if (beta1) val += c[idx];
             c[idx] = val;
@@ -277,10 +362,10 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
 
   if (tid == 0) {
     __threadfence();
-    const int done = atomicAdd(&p.queue[1], 1);
+    const int done = atomicAdd(&p.queue[kRegions], 1);
     if (done == (int)gridDim.x - 1) {
-      atomicExch(&p.queue[0], 0);
-      atomicExch(&p.queue[1], 0);
+      // last workgroup out: every grab has happened, reset for the next launch
+      for (int r = 0; r <= kRegions; ++r) atomicExch(&p.queue[r], 0);
     }
   }
 }

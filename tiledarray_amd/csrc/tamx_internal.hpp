@@ -49,7 +49,7 @@ struct tamx_worklist {
   int32_t* item_prefix = nullptr;  // task_capacity + 1
   int32_t* item_task = nullptr;    // item -> task map, item_capacity entries
   int64_t item_capacity = 0;
-  int32_t* queue = nullptr;        // 4 int32, zero between launches
+  int32_t* queue = nullptr;        // 16 int32 (per-XCD next item, done count), zero between launches
 };
 
 #define TAMX_CHECK_HIP(expr)                                                   \
