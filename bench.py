@@ -37,6 +37,7 @@ def parse():
     p.add_argument("--tile", default=None, help="macro tile BMxBN[xBK[xWGS]] for the grouped GEMM")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--precision", default="f64", help="f64 or bf16xS (S BF16 limbs)")
     return p.parse_args()
 
 
@@ -73,7 +74,8 @@ def main():
         E.make_sparse(ctx, B, thr)
         for X in (A, B):
             sizes[id(X)] = E._sizes_vector(X.trange, X.pool.device)
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, tile=tile)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, tile=tile,
+                      precision=args.precision)
 
     def screening():
         for X in (A, B):
@@ -85,6 +87,8 @@ def main():
     def step(timed=False):
         if sparse:
             screening()
+        if c.nslices:
+            c.split_step()
         c.shape_step()
         c.worklist_step()
         if timed:

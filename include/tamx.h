@@ -94,10 +94,13 @@ enum {
 
 /* One tile pair (A_ik, B_kj) contributing to a task. */
 typedef struct tamx_gemm_pair {
-  int64_t a_off; /* element offset of A_ik (storage k x m when op_a = TRANS) */
-  int64_t b_off; /* element offset of B_kj (storage n x k when op_b = TRANS) */
-  int32_t k;     /* contracted extent */
+  int64_t a_off;  /* element offset of A_ik (storage k x m when op_a = TRANS) */
+  int64_t b_off;  /* element offset of B_kj (storage n x k when op_b = TRANS) */
+  int32_t k;      /* contracted extent */
   int32_t lda, ldb;
+  int32_t a_tile; /* ordinal of A_ik in its operand's tile table (used by the
+                     split-precision path to find the tile's limbs) */
+  int32_t b_tile; /* ordinal of B_kj */
   int32_t reserved;
 } tamx_gemm_pair;
 
@@ -305,6 +308,44 @@ int tamx_grouped_gemm_f32_config(tamx_context* ctx, tamx_worklist* wl,
 int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, float alpha, const float* A,
                           const float* B, float* C, void* stream);
+
+/* ------------------------------------------------ split precision --- */
+
+/* Splitting of FP64 operand tiles into nslices BF16 limbs with one
+ * power-of-two scale per tile (sigma = exponent of the tile's max |a| plus
+ * one): a = 2^sigma * sum_s l_s 2^-8(s+1) + r, |r| <= 2^(sigma-8*nslices-1),
+ * every l_s an integer held exactly in BF16.  The limbs are laid out for the
+ * MFMA: the logical op(A) tile as m x K and op(B) as n x K, K contiguous,
+ * rows padded to 16 and K to 32 with zeros.  No reference counterpart: the
+ * reference multiplies in FP64 only (math/blas.h:171-177); this is the
+ * split-precision path that stands beside it.
+ * The plan lists the operand's tiles by the ordinals the work list's pairs
+ * carry (a_tile / b_tile): host_offsets[t] (< 0: absent), and the stored
+ * (row-major) extents host_rows[t] x host_cols[t].  is_right selects the
+ * right operand layout, op its tamx_op. */
+typedef struct tamx_split_plan tamx_split_plan;
+int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
+                           int64_t ntiles, const int64_t* host_offsets,
+                           const int32_t* host_rows, const int32_t* host_cols,
+                           tamx_split_plan** plan);
+int tamx_split_plan_destroy(tamx_split_plan* plan);
+/* BF16 elements the limb pool of a plan needs. */
+int64_t tamx_split_plan_limb_elems(const tamx_split_plan* plan);
+/* Copies the per-tile scale exponents to the host (synchronous). */
+int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma);
+/* Writes the limbs of every tile of `pool` (device BF16 buffer of
+ * tamx_split_plan_limb_elems elements). */
+int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan,
+                   const double* pool, void* limbs, void* stream);
+/* Grouped GEMM on split operands: per pair the limb products with
+ * s + t < nslices on BF16 MFMA with FP32 accumulation, folded into FP64
+ * (cross-limb sums, the 2^(sigma_A + sigma_B) tile scaling and the C
+ * accumulation in FP64).  Same work list and beta/transpose semantics as
+ * tamx_grouped_gemm_f64. */
+int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
+                            const tamx_split_plan* a_plan, const void* a_limbs,
+                            const tamx_split_plan* b_plan, const void* b_limbs,
+                            double* C, void* stream);
 
 #ifdef __cplusplus
 }

@@ -118,7 +118,8 @@ def test_beta_accumulate_and_empty_worklist(ctx):
     wl = E.WorkList(ctx, 4, 4, 4)
     tasks, pairs, counters = wl.buffers()
     task = _lib.GemmTask(0, m, n, n, 0, 2, _lib.TAMX_TASK_BETA1)
-    pr = (_lib.GemmPair * 2)(_lib.GemmPair(0, 0, k, k, n, 0), _lib.GemmPair(0, 0, k, k, n, 0))
+    pr = (_lib.GemmPair * 2)(_lib.GemmPair(0, 0, k, k, n, 0, 0, 0),
+                             _lib.GemmPair(0, 0, k, k, n, 0, 0, 0))
     cnt = (ctypes.c_int64 * 8)(1, 2, m * n, 0, 0, 0, 0, 0)
     hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
     assert hip.hipMemcpy(tasks, ctypes.byref(task), ctypes.sizeof(task), 1) == 0

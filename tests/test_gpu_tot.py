@@ -1,0 +1,128 @@
+"""Batched tensor-of-tensor contractions (one persistent grouped launch for
+all instances) and their hipGraph capture, against the CPU oracle.
+
+Bars: FP64 integer-valued cells bit-exact; FP32 relative Frobenius error
+<= 1e-5 per instance; shapes (cell norms, scaled norms, result estimate,
+result pattern) bit-exact; every graph replay equal to the eager result.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tiledarray_amd import engine as E
+from tiledarray_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def make(ctx, nbatch, dtype, integer=False, seed=7, fill=0.5):
+    w = W.tot_batch(nbatch=nbatch, dtype=dtype, integer=integer, seed=seed, fill=fill)
+    g = w["grid"]
+    c = E.BatchedContraction(ctx, nbatch, g, g, g, w["a_dims"], w["b_dims"], w["c_dims"],
+                             w["A"], w["a_off"], w["B"], w["b_off"])
+    return w, c
+
+
+def oracle_batch(w, c):
+    """Per instance: shapes and the one-rank contraction, at the product's
+    result offsets; returns (C pool, c_norms, pattern)."""
+    g = w["grid"]
+    Ah, Bh = w["A"].cpu().numpy(), w["B"].cpu().numpy()
+    c_off = c.c_off.cpu().numpy()
+    Cref = np.zeros(c.c_capacity, Ah.dtype)
+    est_all = []
+    thr = c.threshold
+    for b in range(w["nbatch"]):
+        ad, cd, bd = w["a_dims"][b], w["c_dims"][b], w["b_dims"][b]
+        sl = slice(b * g * g, (b + 1) * g * g)
+        a_off, b_off = w["a_off"][sl], w["b_off"][sl]
+        na = oracle.tile_norms(Ah, a_off, np.multiply.outer(ad, cd).reshape(-1)).reshape(g, g)
+        nb = oracle.tile_norms(Bh, b_off, np.multiply.outer(cd, bd).reshape(-1)).reshape(g, g)
+        sa, _ = oracle.scale_tile_norms(na, [ad, cd], thr)
+        sb, _ = oracle.scale_tile_norms(nb, [cd, bd], thr)
+        est, _ = oracle.shape_gemm(sa, sb, 0, 0, g, g, g, oracle.k_sizes([cd]), 1.0, thr)
+        est_all.append(est.reshape(-1))
+        ao = np.where(sa.reshape(-1) >= thr, a_off, -1)
+        bo = np.where(sb.reshape(-1) >= thr, b_off, -1)
+        part, _ = oracle.contract(0, 0, ad, bd, cd, Ah, ao, Bh, bo, c_off[sl], c.c_capacity)
+        for t in range(g * g):
+            o = c_off[sl][t]
+            if o >= 0:
+                n = ad[t // g] * bd[t % g]
+                Cref[o:o + n] = part[o:o + n]
+    return Cref, np.concatenate(est_all), c_off
+
+
+def used_mask(w, c, c_off):
+    g = w["grid"]
+    m = np.zeros(c.c_capacity, bool)
+    for b in range(w["nbatch"]):
+        for t in range(g * g):
+            o = c_off[b * g * g + t]
+            if o >= 0:
+                m[o:o + w["a_dims"][b][t // g] * w["b_dims"][b][t % g]] = True
+    return m
+
+
+def test_tot_fp64_integer_exact_and_shapes(ctx):
+    w, c = make(ctx, 24, torch.float64, integer=True)
+    c.run()
+    torch.cuda.synchronize()
+    Cref, est, c_off = oracle_batch(w, c)
+    np.testing.assert_array_equal(c.c_norms.cpu().numpy(), est)
+    np.testing.assert_array_equal(c_off >= 0, est >= c.threshold)
+    m = used_mask(w, c, c_off)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[m], Cref[m])
+
+
+def test_tot_fp32_tolerance(ctx):
+    w, c = make(ctx, 40, torch.float32, seed=8)
+    c.run()
+    torch.cuda.synchronize()
+    Cref, est, c_off = oracle_batch(w, c)
+    np.testing.assert_array_equal(c_off >= 0, est >= c.threshold)
+    got = c.C.cpu().numpy().astype(np.float64)
+    g = w["grid"]
+    for b in range(w["nbatch"]):
+        idx = []
+        for t in range(g * g):
+            o = c_off[b * g * g + t]
+            if o >= 0:
+                idx.append(np.arange(o, o + w["a_dims"][b][t // g] * w["b_dims"][b][t % g]))
+        if not idx:
+            continue
+        idx = np.concatenate(idx)
+        r = Cref[idx].astype(np.float64)
+        assert np.linalg.norm(got[idx] - r) <= 1e-5 * np.linalg.norm(r), b
+
+
+def test_tot_graph_replays_match_eager(ctx):
+    w, c = make(ctx, 64, torch.float32, seed=9)
+    graph = c.capture()
+    for rep in range(3):
+        # eager work between replays: new operand data, then the eager result
+        w["A"].mul_(1.5 if rep % 2 == 0 else -0.5)
+        w["B"].add_(0.25)
+        torch.cuda.synchronize()
+        c.run()
+        torch.cuda.synchronize()
+        eager = c.C.clone()
+        eager_off = c.c_off.clone()
+        c.C.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(c.c_off, eager_off)
+        m = torch.as_tensor(used_mask(w, c, eager_off.cpu().numpy()), device=c.C.device)
+        assert torch.equal(c.C[m], eager[m]), rep
+
+
+def test_tot_empty_instances(ctx):
+    # instances with no cells at all produce no tasks
+    w, c = make(ctx, 8, torch.float64, integer=True, fill=0.0)
+    c.run()
+    torch.cuda.synchronize()
+    cnt = c.counters()
+    assert cnt["ntasks"] == 0 and cnt["npairs"] == 0
+    assert (c.c_off.cpu().numpy() < 0).all()

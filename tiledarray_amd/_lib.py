@@ -32,7 +32,8 @@ class GemmTask(ctypes.Structure):
 
 class GemmPair(ctypes.Structure):
     _fields_ = [("a_off", ctypes.c_int64), ("b_off", ctypes.c_int64), ("k", ctypes.c_int32),
-                ("lda", ctypes.c_int32), ("ldb", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("lda", ctypes.c_int32), ("ldb", ctypes.c_int32), ("a_tile", ctypes.c_int32),
+                ("b_tile", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ContractionDesc(ctypes.Structure):
@@ -122,6 +123,12 @@ SIGNATURES = {
     "tamx_grouped_gemm_f64_config": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32_config": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp],
+    "tamx_split_plan_create": [_vp, _i, _i, _i, _i64, _vp, _vp, _vp, _vp],
+    "tamx_split_plan_destroy": [_vp],
+    "tamx_split_plan_limb_elems": [_vp],
+    "tamx_split_plan_sigmas": [_vp, _vp],
+    "tamx_split_f64": [_vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_split": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {
     "tamx_gemm_config_count": ctypes.c_int,
@@ -129,6 +136,7 @@ _RESTYPES = {
     "tamx_last_error": ctypes.c_char_p,
     "tamx_version": ctypes.c_char_p,
     "tamx_permute_plan_volume": _i64,
+    "tamx_split_plan_limb_elems": _i64,
 }
 
 _lib = None

@@ -378,6 +378,8 @@ __global__ void wl_fill_kernel(WlParams p) {
     pr.k = kk;
     pr.lda = d.op_a ? m : kk;
     pr.ldb = d.op_b ? kk : n;
+    pr.a_tile = (int32_t)ai;
+    pr.b_tile = (int32_t)bi;
     pr.reserved = 0;
     p.pairs[q++] = pr;
   }

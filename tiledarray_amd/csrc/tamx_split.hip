@@ -1,0 +1,664 @@
+// Split-precision contraction: FP64 tiles are split into BF16 limbs with one
+// power-of-two scale per tile, the limb products run on BF16 MFMA with FP32
+// accumulation, and every FP32 partial sum is folded into FP64 before it can
+// round.
+//
+// Splitting (one pass per operand, tamx_split_f64):
+//   sigma = exponent of the tile's max |a| plus one, so x = a * 2^-sigma is in
+//   (-1, 1);  l_0 = rint(256 x), x_1 = 256 x - l_0, l_1 = rint(256 x_1), ...
+//   Every step is exact in FP64; |l_0| <= 256, |l_s| <= 128 for s >= 1, so
+//   every limb is an integer that BF16 holds exactly, and
+//   a = 2^sigma * sum_s l_s 2^-8(s+1) + r with |r| <= 2^(sigma - 8S - 1).
+//   Limbs are written in the layout the MFMA wants: the logical op(A) tile as
+//   rows x K and op(B) as columns x K, K contiguous, rows padded to 16 and K
+//   to 32 with zeros, so the GEMM needs no bounds checks and 16-byte loads.
+//
+// Multiplying (tamx_grouped_gemm_split): for a tile pair the limb products
+// with s + t < S (the triangular set, S(S+1)/2 MFMA products) are
+// accumulated per level g = s + t in FP32.  Products are integers below
+// 2^16 and a level holds at most 2^16 + 2^15 + 2^14 + ... per k, so 128
+// consecutive k stay below 2^24 and every FP32 sum is exact.  Every 128 k and
+// at each pair boundary the levels are folded in FP64:
+//   C += 2^(sigma_A + sigma_B) * sum_g 2^-8(g+2) * P_g.
+// The result error is the truncation of the limb expansion (about
+// 2^-8S relative to the tiles' maxima) plus FP64 rounding.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "tamx_internal.hpp"
+#include "tamx_scan.hpp"
+
+namespace {
+
+struct SplitTile {
+  int64_t src_off;   // element offset of the FP64 tile
+  int64_t limb_off;  // element offset of limb plane 0 in the limb pool
+  int32_t rows, cols;  // stored FP64 tile (row-major)
+  int32_t out_rows;    // logical rows of the limb layout (m for A, n for B)
+  int32_t out_k;       // logical K
+  int32_t rows_pad;    // out_rows rounded up to 16
+  int32_t k_pad;       // out_k rounded up to 32 (row stride of a plane)
+  int32_t transpose;   // limb (row, k) reads stored (k, row)
+  int32_t sigma;       // scale exponent, written by the max pass
+};
+
+}  // namespace
+
+struct tamx_split_plan {
+  tamx_context* ctx = nullptr;
+  int64_t ntiles = 0;
+  int32_t nslices = 0;
+  int64_t limb_elems = 0;   // size of the limb pool in bf16 elements
+  int64_t nblocks = 0;      // 32 x 32 output blocks over all tiles
+  SplitTile* tiles = nullptr;        // device
+  int64_t* block_prefix = nullptr;   // device, ntiles + 1
+  unsigned long long* maxbits = nullptr;  // device, per-tile max |a| bits
+};
+
+namespace {
+
+constexpr int kSplitTS = 32;
+
+__device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t item) {
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// max |a| per tile over 32 x 32 blocks of the stored tile (the same block
+// enumeration as the split pass); positive doubles order like their bits
+__global__ __launch_bounds__(256) void split_max_kernel(
+    const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
+    const int64_t* __restrict__ prefix, int64_t ntiles,
+    unsigned long long* __restrict__ maxbits) {
+  __shared__ double red[256];
+  const int64_t nblocks = prefix[ntiles];
+  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
+    const int64_t ti = find_tile(prefix, ntiles, item);
+    const SplitTile& t = tiles[ti];
+    const int64_t local = item - prefix[ti];
+    const int nbk = t.k_pad / kSplitTS;
+    const int br = (int)(local / nbk), bk = (int)(local % nbk);
+    double m = 0.0;
+    for (int e = threadIdx.x; e < kSplitTS * kSplitTS; e += 256) {
+      const int r = br * kSplitTS + e / kSplitTS, k = bk * kSplitTS + e % kSplitTS;
+      if (r < t.out_rows && k < t.out_k) {
+        const int64_t idx = t.transpose ? (int64_t)k * t.cols + r : (int64_t)r * t.cols + k;
+        m = fmax(m, fabs(pool[t.src_off + idx]));
+      }
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] > 0.0)
+      atomicMax(&maxbits[ti], (unsigned long long)__double_as_longlong(red[0]));
+    __syncthreads();
+  }
+}
+
+__global__ void split_sigma_kernel(SplitTile* tiles, const unsigned long long* maxbits,
+                                   int64_t ntiles) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const double m = __longlong_as_double((long long)maxbits[t]);
+  // x = a * 2^-sigma lies in (-1, 1): m < 2^(ilogb(m) + 1)
+  tiles[t].sigma = m > 0.0 ? ilogb(m) + 1 : 0;
+}
+
+// Writes the S limb planes of every 32 x 32 block of the limb layout.  A
+// transposed read goes through LDS so both the FP64 reads and the BF16
+// writes are contiguous.
+template <int S>
+__global__ __launch_bounds__(256) void split_limbs_kernel(
+    const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
+    const int64_t* __restrict__ prefix, int64_t ntiles, __bf16* __restrict__ limbs) {
+#pragma clang fp contract(off)
+  __shared__ double tileb[kSplitTS][kSplitTS + 1];
+  const int64_t nblocks = prefix[ntiles];
+  const int tx = threadIdx.x % kSplitTS, ty = threadIdx.x / kSplitTS;  // 32 x 8
+  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
+    const int64_t ti = find_tile(prefix, ntiles, item);
+    const SplitTile& t = tiles[ti];
+    const int64_t local = item - prefix[ti];
+    const int nbk = t.k_pad / kSplitTS;
+    const int br = (int)(local / nbk), bk = (int)(local % nbk);
+    const int r0 = br * kSplitTS, k0 = bk * kSplitTS;
+    // stage the source block as tileb[row][k] of the limb layout
+    for (int y = ty; y < kSplitTS; y += 8) {
+      double v = 0.0;
+      if (t.transpose) {
+        // stored (k, row): read along row (contiguous), store transposed
+        const int k = k0 + y, r = r0 + tx;
+        if (r < t.out_rows && k < t.out_k) v = pool[t.src_off + (int64_t)k * t.cols + r];
+        tileb[tx][y] = v;
+      } else {
+        const int r = r0 + y, k = k0 + tx;
+        if (r < t.out_rows && k < t.out_k) v = pool[t.src_off + (int64_t)r * t.cols + k];
+        tileb[y][tx] = v;
+      }
+    }
+    __syncthreads();
+    const int64_t plane = (int64_t)t.rows_pad * t.k_pad;
+    for (int y = ty; y < kSplitTS; y += 8) {
+      const int r = r0 + y;
+      if (r >= t.rows_pad) continue;
+      double x = ldexp(tileb[y][tx], -t.sigma);
+      const int64_t o = t.limb_off + (int64_t)r * t.k_pad + k0 + tx;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const double xs = x * 256.0;
+        const double l = rint(xs);
+        limbs[o + s * plane] = (__bf16)(float)l;
+        x = xs - l;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int S>
+void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_plan* p,
+                  __bf16* limbs) {
+  split_limbs_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles, limbs);
+}
+
+// ------------------------------------------------------------ the GEMM --
+
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+struct SplitParams {
+  const bf8_t* __restrict__ LA;  // limb pools viewed as 8-element vectors
+  const bf8_t* __restrict__ LB;
+  const SplitTile* __restrict__ ta;
+  const SplitTile* __restrict__ tb;
+  double* __restrict__ C;
+  const tamx_gemm_task* __restrict__ tasks;
+  const tamx_gemm_pair* __restrict__ pairs;
+  const int64_t* __restrict__ counters;
+  const int32_t* __restrict__ prefix;
+  const int32_t* __restrict__ item_task;
+  int32_t* queue;
+  int64_t task_capacity;
+  int64_t item_capacity;
+  double alpha;
+};
+
+constexpr int kThreads = 256;
+constexpr int kRegions = 8;
+constexpr int SBM = 64, SBN = 64, SBK = 32;
+constexpr int kFoldChunks = 4;  // fold FP32 levels into FP64 every 128 k
+
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline int64_t uni64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__global__ __launch_bounds__(1024) void split_plan_items_kernel(
+    const tamx_gemm_task* __restrict__ tasks, const int64_t* __restrict__ counters,
+    int32_t* __restrict__ prefix, int32_t* __restrict__ item_task, int64_t capacity,
+    int64_t item_capacity) {
+  __shared__ int32_t part[1024];
+  int64_t n = counters[TAMX_WL_NTASKS];
+  if (n < 0 || n > capacity || counters[TAMX_WL_STATUS] != 0) n = 0;
+  auto count = [&](int64_t i) -> int32_t {
+    const tamx_gemm_task t = tasks[i];
+    return ((t.m + SBM - 1) / SBM) * ((t.n + SBN - 1) / SBN);
+  };
+  const int32_t total = tamx::block_exclusive_scan<1024, int32_t>(
+      n, part, count, [&](int64_t i, int32_t v) { prefix[i] = v; });
+  if (threadIdx.x == 0) prefix[n] = total;
+  if (total <= item_capacity) {
+    const int64_t seg = (n + 1023) / 1024;
+    const int64_t b = min<int64_t>(n, threadIdx.x * seg), e = min<int64_t>(n, b + seg);
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t q0 = prefix[i], q1 = q0 + count(i);
+      for (int32_t q = q0; q < q1; ++q) item_task[q] = (int32_t)i;
+    }
+  }
+}
+
+// LDS layout of one operand chunk: S planes x 64 rows x 32 k of BF16, as
+// 16-byte units (8 k each); unit q of row r sits at r * 4 + (q ^ (r & 3)) so
+// the 16 rows a quarter-wave reads fall in different banks.
+__device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ (r & 3)); }
+
+template <int S>
+__global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
+  constexpr int NG = S;  // levels g = s + t < S
+  __shared__ bf8_t sA[2][S][SBM * 4];
+  __shared__ bf8_t sB[2][S][SBN * 4];
+  __shared__ tamx_gemm_pair s_pairs[64];
+  __shared__ int s_item;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = uni(tid >> 6);
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lrow = lane & 15, lq = lane >> 4;  // MFMA: row lrow, k unit lq
+  // staging: thread loads row (tid / 4), unit (tid % 4) of every plane
+  const int st_r = tid >> 2, st_q = tid & 3;
+
+  int64_t ntasks64 = p.counters[TAMX_WL_NTASKS];
+  if (ntasks64 < 0 || ntasks64 > p.task_capacity || p.counters[TAMX_WL_STATUS] != 0)
+    ntasks64 = 0;
+  const int32_t ntasks = uni((int32_t)ntasks64);
+  const int32_t total = uni(p.prefix[ntasks > 0 ? ntasks : 0]);
+  const int home = blockIdx.x % kRegions;
+  int tried = 0;
+  for (;;) {
+    if (tid == 0) {
+      int it = -1;
+      while (tried < kRegions) {
+        const int r = (home + tried) % kRegions;
+        const int rb = (int)((int64_t)total * r / kRegions);
+        const int re = (int)((int64_t)total * (r + 1) / kRegions);
+        const int q = rb < re ? atomicAdd(&p.queue[r], 1) : 0;
+        if (rb + q < re) {
+          it = rb + q;
+          break;
+        }
+        ++tried;
+      }
+      s_item = it;
+    }
+    __syncthreads();
+    const int item = uni(s_item);
+    __syncthreads();
+    if (item < 0) break;
+    int lo;
+    if (total <= p.item_capacity) {
+      lo = uni(p.item_task[item]);
+    } else {
+      lo = 0;
+      int hi = ntasks - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uni(p.prefix[mid]) <= item) lo = mid; else hi = mid - 1;
+      }
+    }
+    const tamx_gemm_task* tp = p.tasks + lo;
+    const int64_t t_coff = uni64(tp->c_off);
+    const int t_m = uni(tp->m), t_n = uni(tp->n), t_ldc = uni(tp->ldc);
+    const int t_pb = uni(tp->pair_begin), t_pc = uni(tp->pair_count);
+    const int t_flags = uni(tp->flags);
+    const int local = item - uni(p.prefix[lo]);
+    const int tiles_n = (t_n + SBN - 1) / SBN;
+    const int r0 = (local / tiles_n) * SBM, c0 = (local % tiles_n) * SBN;
+    const int mrem = min(SBM, t_m - r0), nrem = min(SBN, t_n - c0);
+    const bool wave_active = (wr < mrem) && (wc < nrem);
+
+    d4_t acc[2][2];
+    f4_t lvl[NG][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0.0f;
+      }
+
+    const int pend = t_pb + t_pc;
+    const bool pairs_in_lds = t_pc <= 64;
+    if (pairs_in_lds && t_pc > 0) {
+      const int32_t* src = reinterpret_cast<const int32_t*>(p.pairs + t_pb);
+      int32_t* dst = reinterpret_cast<int32_t*>(s_pairs);
+      constexpr int W = sizeof(tamx_gemm_pair) / sizeof(int32_t);
+      for (int x = tid; x < t_pc * W; x += kThreads) dst[x] = src[x];
+    }
+    __syncthreads();
+
+    // load cursor: pair lp, chunk lk0
+    int lp = t_pb, lk0 = 0, lkp = 0;
+    int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0;
+    auto set_pair = [&](int pi) {
+      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
+      const int at = uni(pp.a_tile), bt = uni(pp.b_tile);
+      const SplitTile* A = p.ta + at;
+      const SplitTile* B = p.tb + bt;
+      lkp = uni(A->k_pad);
+      // element offsets of this thread's staging row in plane 0, in units of 8
+      la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> 3;
+      lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> 3;
+      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> 3;
+      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> 3;
+    };
+    if (lp < pend) set_pair(lp);
+    auto advance = [&]() {
+      lk0 += SBK;
+      if (lk0 >= lkp) {
+        ++lp;
+        lk0 = 0;
+        if (lp < pend) set_pair(lp);
+      }
+    };
+    bf8_t ra[S], rb[S];
+    auto load_chunk = [&]() {
+      const int64_t q = (lk0 >> 3) + st_q;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        ra[s] = p.LA[la_row + s * la_plane + q];
+        rb[s] = p.LB[lb_row + s * lb_plane + q];
+      }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        sA[buf][s][lds_unit(st_r, st_q)] = ra[s];
+        sB[buf][s][lds_unit(st_r, st_q)] = rb[s];
+      }
+    };
+    auto compute_chunk = [&](int buf) {
+      if (!wave_active) return;
+      bf8_t af[2][S], bf[2][S];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int s = 0; s < S; ++s) af[i][s] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int s = 0; s < S; ++s) bf[j][s] = sB[buf][s][lds_unit(wc + j * 16 + lrow, lq)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int t = 0; t < S - s; ++t)
+              lvl[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  af[i][s], bf[j][t], lvl[s + t][i][j], 0, 0, 0);
+    };
+    // fold the FP32 levels of the pair being computed (exponent e) into FP64
+    auto fold = [&](int e) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double tsum = 0.0;
+#pragma unroll
+            for (int g = NG - 1; g >= 0; --g) {
+              tsum += ldexp((double)lvl[g][i][j][v], -8 * (g + 2));
+              lvl[g][i][j][v] = 0.0f;
+            }
+            acc[i][j][v] += ldexp(tsum, e);
+          }
+    };
+
+    // compute cursor: pair cp, chunks done since the last fold
+    int cp = t_pb, ck0 = 0, ckp = 0, ce = 0, since_fold = 0;
+    auto set_compute_pair = [&](int pi) {
+      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
+      const SplitTile* A = p.ta + uni(pp.a_tile);
+      const SplitTile* B = p.tb + uni(pp.b_tile);
+      ckp = uni(A->k_pad);
+      ce = uni(A->sigma) + uni(B->sigma);
+    };
+    if (cp < pend) set_compute_pair(cp);
+    // after computing one chunk: fold at a pair end or every kFoldChunks
+    auto after_compute = [&]() {
+      ++since_fold;
+      ck0 += SBK;
+      const bool pair_end = ck0 >= ckp;
+      if (pair_end || since_fold == kFoldChunks) {
+        if (wave_active) fold(ce);
+        since_fold = 0;
+      }
+      if (pair_end) {
+        ++cp;
+        ck0 = 0;
+        if (cp < pend) set_compute_pair(cp);
+      }
+    };
+
+    if (lp < pend) {
+      load_chunk();
+      advance();
+      store_chunk(0);
+      __syncthreads();
+      int buf = 0;
+      for (;;) {
+        const bool has_next = lp < pend;
+        if (has_next) {
+          load_chunk();
+          advance();
+        }
+        compute_chunk(buf);
+        after_compute();
+        if (!has_next) break;
+        store_chunk(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
+    }
+
+    const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
+    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
+    double* __restrict__ c = p.C + t_coff;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // f32-output 16x16 MFMA layout: row 4 * (lane / 16) + v, column lane % 16
+          const int row = wr + i * 16 + lq * 4 + v;
+          const int col = wc + j * 16 + lrow;
+          if (row < mrem && col < nrem) {
+            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
+                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
+            double val = p.alpha * acc[i][j][v];
+            if (beta1) val += c[idx];
+            c[idx] = val;
+          }
+        }
+  }
+
+  if (tid == 0) {
+    __threadfence();
+    const int done = atomicAdd(&p.queue[kRegions], 1);
+    if (done == (int)gridDim.x - 1) {
+      for (int r = 0; r <= kRegions; ++r) atomicExch(&p.queue[r], 0);
+    }
+  }
+}
+
+template <int S>
+int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm,
+                      hipStream_t s) {
+  auto kern = split_gemm_kernel<S>;
+  int occ = 0;
+  TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
+  if (occ < 1) occ = 1;
+  split_plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, wl->item_prefix,
+                                             wl->item_task, wl->task_capacity,
+                                             wl->item_capacity);
+  TAMX_LAUNCH_CHECK();
+  kern<<<ctx->num_cus * occ, kThreads, 0, s>>>(prm);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
+                           int64_t ntiles, const int64_t* host_offsets,
+                           const int32_t* host_rows, const int32_t* host_cols,
+                           tamx_split_plan** out) {
+  TAMX_REQUIRE(out != nullptr, "null output pointer");
+  *out = nullptr;
+  TAMX_REQUIRE(nslices >= 2 && nslices <= 6, "nslices must be in [2, 6]");
+  TAMX_REQUIRE(op == 0 || op == 1, "op must be TAMX_NO_TRANS or TAMX_TRANS");
+  TAMX_REQUIRE(ntiles >= 0, "negative tile count");
+  TAMX_REQUIRE(ntiles == 0 || (host_offsets && host_rows && host_cols), "null tile table");
+  if (int rc = tamx::activate(ctx)) return rc;
+  // A (left): limb rows = m.  NoTrans stores m x k (no transpose); Trans
+  // stores k x m.  B (right): limb rows = n.  NoTrans stores k x n
+  // (transpose); Trans stores n x k.
+  const int transpose = is_right ? (op == 0) : (op == 1);
+  std::vector<SplitTile> tiles(ntiles);
+  std::vector<int64_t> prefix(ntiles + 1, 0);
+  int64_t off = 0, blocks = 0, max_kpad = 32;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    SplitTile& st = tiles[t];
+    st = SplitTile{};
+    st.src_off = host_offsets[t];
+    st.rows = host_rows[t];
+    st.cols = host_cols[t];
+    st.transpose = transpose;
+    prefix[t] = blocks;
+    if (st.src_off < 0) {  // absent tile: no limbs
+      st.limb_off = 0;
+      continue;
+    }
+    TAMX_REQUIRE(st.rows >= 0 && st.cols >= 0, "negative tile extent");
+    st.out_rows = transpose ? st.cols : st.rows;
+    st.out_k = transpose ? st.rows : st.cols;
+    st.rows_pad = (st.out_rows + 15) / 16 * 16;
+    st.k_pad = std::max(32, (st.out_k + 31) / 32 * 32);
+    st.limb_off = off;
+    off += (int64_t)nslices * st.rows_pad * st.k_pad;
+    max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
+    blocks += (int64_t)((st.rows_pad + kSplitTS - 1) / kSplitTS) * (st.k_pad / kSplitTS);
+  }
+  prefix[ntiles] = blocks;
+  auto* plan = new (std::nothrow) tamx_split_plan();
+  if (!plan) return TAMX_ERR_OOM;
+  plan->ctx = ctx;
+  plan->ntiles = ntiles;
+  plan->nslices = nslices;
+  // slack past the last tile: a 64-row macro tile may read up to 63 rows
+  // beyond a tile's padded rows (those rows only reach unstored results)
+  plan->limb_elems = off + 64 * max_kpad * nslices;
+  plan->nblocks = blocks;
+  hipError_t e = hipMalloc(&plan->tiles, std::max<size_t>(1, tiles.size()) * sizeof(SplitTile));
+  if (e == hipSuccess) e = hipMalloc(&plan->block_prefix, prefix.size() * sizeof(int64_t));
+  if (e == hipSuccess)
+    e = hipMalloc(&plan->maxbits, std::max<size_t>(1, tiles.size()) * sizeof(unsigned long long));
+  if (e == hipSuccess && !tiles.empty())
+    e = hipMemcpy(plan->tiles, tiles.data(), tiles.size() * sizeof(SplitTile), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->block_prefix, prefix.data(), prefix.size() * sizeof(int64_t),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tamx::set_error(std::string("split plan: ") + hipGetErrorString(e));
+    if (plan->tiles) (void)hipFree(plan->tiles);
+    if (plan->block_prefix) (void)hipFree(plan->block_prefix);
+    if (plan->maxbits) (void)hipFree(plan->maxbits);
+    delete plan;
+    return e == hipErrorOutOfMemory ? TAMX_ERR_OOM : TAMX_ERR_DEVICE;
+  }
+  *out = plan;
+  return TAMX_OK;
+}
+
+int tamx_split_plan_destroy(tamx_split_plan* plan) {
+  if (!plan) return TAMX_OK;
+  if (plan->tiles) (void)hipFree(plan->tiles);
+  if (plan->block_prefix) (void)hipFree(plan->block_prefix);
+  if (plan->maxbits) (void)hipFree(plan->maxbits);
+  delete plan;
+  return TAMX_OK;
+}
+
+int64_t tamx_split_plan_limb_elems(const tamx_split_plan* plan) {
+  return plan ? plan->limb_elems : 0;
+}
+
+int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma) {
+  TAMX_REQUIRE(plan != nullptr && host_sigma != nullptr, "null argument");
+  std::vector<SplitTile> tiles(plan->ntiles);
+  if (!tiles.empty())
+    TAMX_CHECK_HIP(hipMemcpy(tiles.data(), plan->tiles, tiles.size() * sizeof(SplitTile),
+                             hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < tiles.size(); ++t) host_sigma[t] = tiles[t].sigma;
+  return TAMX_OK;
+}
+
+int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan, const double* pool,
+                   void* limbs, void* stream) {
+  TAMX_REQUIRE(plan != nullptr && limbs != nullptr, "null argument");
+  if (plan->ntiles == 0) return TAMX_OK;
+  TAMX_REQUIRE(pool != nullptr || plan->nblocks == 0, "null pool");
+  if (int rc = tamx::activate(ctx)) return rc;
+  hipStream_t s = tamx::as_stream(stream);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(plan->nblocks, ctx->num_cus * 16));
+  zero_u64_kernel<<<(unsigned)((plan->ntiles + 255) / 256), 256, 0, s>>>(plan->maxbits, plan->ntiles);
+  TAMX_LAUNCH_CHECK();
+  if (plan->nblocks > 0) {
+    split_max_kernel<<<grid, 256, 0, s>>>(pool, plan->tiles, plan->block_prefix, plan->ntiles,
+                                          plan->maxbits);
+    TAMX_LAUNCH_CHECK();
+  }
+  split_sigma_kernel<<<(unsigned)((plan->ntiles + 255) / 256), 256, 0, s>>>(
+      plan->tiles, plan->maxbits, plan->ntiles);
+  TAMX_LAUNCH_CHECK();
+  if (plan->nblocks > 0) {
+    __bf16* L = static_cast<__bf16*>(limbs);
+    switch (plan->nslices) {
+      case 2: launch_split<2>(grid, s, pool, plan, L); break;
+      case 3: launch_split<3>(grid, s, pool, plan, L); break;
+      case 4: launch_split<4>(grid, s, pool, plan, L); break;
+      case 5: launch_split<5>(grid, s, pool, plan, L); break;
+      default: launch_split<6>(grid, s, pool, plan, L); break;
+    }
+    TAMX_LAUNCH_CHECK();
+  }
+  return TAMX_OK;
+}
+
+int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
+                            const tamx_split_plan* a_plan, const void* a_limbs,
+                            const tamx_split_plan* b_plan, const void* b_limbs,
+                            double* C, void* stream) {
+  TAMX_REQUIRE(wl && a_plan && b_plan && a_limbs && b_limbs, "null argument");
+  TAMX_REQUIRE(a_plan->nslices == b_plan->nslices, "operands split into different limb counts");
+  if (int rc = tamx::activate(ctx)) return rc;
+  SplitParams prm{static_cast<const bf8_t*>(a_limbs),
+                  static_cast<const bf8_t*>(b_limbs),
+                  a_plan->tiles,
+                  b_plan->tiles,
+                  C,
+                  wl->tasks,
+                  wl->pairs,
+                  wl->counters,
+                  wl->item_prefix,
+                  wl->item_task,
+                  wl->queue,
+                  wl->task_capacity,
+                  wl->item_capacity,
+                  alpha};
+  hipStream_t s = tamx::as_stream(stream);
+  switch (a_plan->nslices) {
+    case 2: return launch_split_gemm<2>(ctx, wl, prm, s);
+    case 3: return launch_split_gemm<3>(ctx, wl, prm, s);
+    case 4: return launch_split_gemm<4>(ctx, wl, prm, s);
+    case 5: return launch_split_gemm<5>(ctx, wl, prm, s);
+    default: return launch_split_gemm<6>(ctx, wl, prm, s);
+  }
+}
+
+}  // extern "C"

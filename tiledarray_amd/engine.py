@@ -143,6 +143,19 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+class _SplitOperand:
+    def __init__(self, ptr, limbs, ntiles):
+        self.ptr, self.limbs, self.ntiles = ptr, limbs, ntiles
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_split_plan_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
 # ------------------------------------------------------------- shapes ------
 
 def tile_norms(ctx, arr, out=None):
@@ -300,6 +313,56 @@ class Contraction:
         self.wl = WorkList(ctx, grid, max(1, grid * self.Kt), grid)
         self.pair_threshold = float(pair_threshold)
         self._desc = self._make_desc()
+        self.nslices = 0
+        if precision.startswith("bf16x"):
+            if self.dtype != torch.float64:
+                raise ValueError("split precision takes FP64 operands")
+            self.nslices = int(precision[5:])
+            self.a_split = self._split_plan(self.A, False, self.op_a)
+            self.b_split = self._split_plan(self.B, True, self.op_b)
+        elif precision not in ("f64", "f32"):
+            raise ValueError(f"unknown precision {precision!r}")
+
+    def _split_plan(self, X, is_right, op):
+        """Split plan over the operand's fused tile grid (ordinals as in the
+        work list's pairs) and its BF16 limb pool."""
+        Mt, Nt, Kt = self.Mt, self.Nt, self.Kt
+        if not is_right:
+            outer, rows_first = self.mext_h, not op  # NoTrans: (m, k)
+            go, gi = Mt, Kt
+        else:
+            outer, rows_first = self.next_h, bool(op)  # Trans: (n, k)
+            go, gi = Nt, Kt
+        if rows_first:  # grid (outer x inner)
+            rows = np.repeat(outer, gi)
+            cols = np.tile(self.kext_h, go)
+        else:  # grid (inner x outer)
+            rows = np.repeat(self.kext_h, go)
+            cols = np.tile(outer, gi)
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        offs = np.ascontiguousarray(X.offsets, np.int64)
+        p = ctypes.c_void_p()
+        call("tamx_split_plan_create", self.ctx.ptr, self.nslices, 1 if is_right else 0,
+             int(op), offs.size, offs.ctypes.data, rows.ctypes.data, cols.ctypes.data,
+             ctypes.byref(p))
+        n = int(lib().tamx_split_plan_limb_elems(p))
+        limbs = torch.empty(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
+        return _SplitOperand(p, limbs, offs.size)
+
+    def split_step(self):
+        """Split both operands into BF16 limbs (part of the split path)."""
+        for X, sp in ((self.A, self.a_split), (self.B, self.b_split)):
+            call("tamx_split_f64", self.ctx.ptr, sp.ptr, X.pool.data_ptr(), sp.limbs.data_ptr(),
+                 self.ctx.stream())
+
+    def sigmas(self):
+        out = []
+        for sp in (self.a_split, self.b_split):
+            buf = np.zeros(sp.ntiles, np.int32)
+            call("tamx_split_plan_sigmas", sp.ptr, buf.ctypes.data)
+            out.append(buf)
+        return out
 
     def _make_desc(self):
         d = _lib.ContractionDesc()
@@ -361,11 +424,15 @@ class Contraction:
             call("tamx_grouped_gemm_f32_config", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
                  ctypes.c_float(self.alpha), A, B, C, cfg, s)
         else:
-            raise ValueError(f"unknown precision {self.precision!r}")
+            call("tamx_grouped_gemm_split", self.ctx.ptr, self.wl.ptr, ctypes.c_double(self.alpha),
+                 self.a_split.ptr, self.a_split.limbs.data_ptr(), self.b_split.ptr,
+                 self.b_split.limbs.data_ptr(), C, s)
 
     def run(self):
         if self.permute_operands:
             self.permute_step()
+        if self.nslices:
+            self.split_step()
         self.shape_step()
         self.worklist_step()
         self.gemm_step()
@@ -414,3 +481,130 @@ def contract(ctx, target, A, left, B, right, **kw):
     c = Contraction(ctx, target, A, left, B, right, **kw)
     c.run()
     return c.result(), c.counters()
+
+
+class BatchedContraction:
+    """nbatch independent block-sparse products C_n = A_n * B_n in one
+    persistent grouped launch: the batched (fused-index) product of
+    tile_op/batched_contract_reduce.h:48-240 and the tensor-of-tensor
+    element-op GEMM of tensor/tensor.h:3226-3680 (outer cells are tiles, the
+    inner cell matrices are their data).
+
+    Every instance has its own Mt x Kt and Kt x Nt cell grids (the same
+    grid sizes) and its own inner extents; shapes are built from cell norms
+    (float, inverse-volume scaled) and screened with `threshold`, exactly as
+    for a single contraction, batched."""
+
+    def __init__(self, ctx, nbatch, Mt, Nt, Kt, mdims, ndims, kdims, A, a_off, B, b_off,
+                 threshold=FLOAT_EPS, alpha=1.0, tile=None):
+        self.ctx = ctx
+        dev = A.device
+        self.nbatch, self.Mt, self.Nt, self.Kt = int(nbatch), int(Mt), int(Nt), int(Kt)
+        self.dtype = A.dtype
+        self.A, self.B = A, B
+        self.alpha = float(alpha)
+        self.threshold = float(np.float32(threshold))
+        md = np.asarray(mdims, np.int64).reshape(nbatch, Mt)
+        nd = np.asarray(ndims, np.int64).reshape(nbatch, Nt)
+        kd = np.asarray(kdims, np.int64).reshape(nbatch, Kt)
+        self.mdims, self.ndims, self.kdims = md, nd, kd
+        self.a_off_h = np.ascontiguousarray(a_off, np.int64)
+        self.b_off_h = np.ascontiguousarray(b_off, np.int64)
+        self.d_a_off = torch.as_tensor(self.a_off_h, device=dev)
+        self.d_b_off = torch.as_tensor(self.b_off_h, device=dev)
+        a_vol = (md[:, :, None] * kd[:, None, :]).reshape(-1)
+        b_vol = (kd[:, :, None] * nd[:, None, :]).reshape(-1)
+        self.a_plan = NormPlan(ctx, self.a_off_h, a_vol)
+        self.b_plan = NormPlan(ctx, self.b_off_h, b_vol)
+        self.a_norms = torch.zeros(nbatch * Mt * Kt, dtype=torch.float32, device=dev)
+        self.b_norms = torch.zeros(nbatch * Kt * Nt, dtype=torch.float32, device=dev)
+        self.c_norms = torch.zeros(nbatch * Mt * Nt, dtype=torch.float32, device=dev)
+        f32 = np.float32
+        self.a_sizes = torch.as_tensor(np.concatenate([md, kd], axis=1).astype(f32).reshape(-1),
+                                       device=dev)
+        self.b_sizes = torch.as_tensor(np.concatenate([kd, nd], axis=1).astype(f32).reshape(-1),
+                                       device=dev)
+        self.k_sizes = torch.as_tensor(kd.astype(f32).reshape(-1), device=dev)
+        self.d_mext = torch.as_tensor(md.astype(np.int32).reshape(-1), device=dev)
+        self.d_next = torch.as_tensor(nd.astype(np.int32).reshape(-1), device=dev)
+        self.d_kext = torch.as_tensor(kd.astype(np.int32).reshape(-1), device=dev)
+        self.c_capacity = int(sum(int(md[b].sum()) * int(nd[b].sum()) for b in range(nbatch)))
+        self.C = torch.empty(max(1, self.c_capacity), dtype=self.dtype, device=dev)
+        self.c_off = torch.empty(nbatch * Mt * Nt, dtype=torch.int64, device=dev)
+        grid = nbatch * Mt * Nt
+        self.wl = WorkList(ctx, grid, grid * Kt, grid)
+        d = _lib.ContractionDesc()
+        d.op_a, d.op_b = 0, 0
+        d.Mt, d.Nt, d.Kt, d.nbatch = self.Mt, self.Nt, self.Kt, self.nbatch
+        d.mext, d.next, d.kext = (self.d_mext.data_ptr(), self.d_next.data_ptr(),
+                                  self.d_kext.data_ptr())
+        d.a_off, d.b_off = self.d_a_off.data_ptr(), self.d_b_off.data_ptr()
+        d.a_norms, d.b_norms = self.a_norms.data_ptr(), self.b_norms.data_ptr()
+        d.a_threshold = d.b_threshold = d.c_threshold = self.threshold
+        d.c_norms = self.c_norms.data_ptr()
+        d.pair_threshold = 0.0
+        d.store_trans = 0
+        d.c_off = self.c_off.data_ptr()
+        d.c_capacity = self.c_capacity
+        self._desc = d
+        self.tile_cfg = None if tile is None else _lib.GemmConfig(*(tuple(tile) + (16, 0))[:4])
+
+    def shape_step(self):
+        """Cell norms, inverse-volume scaling, batched result estimate."""
+        s = self.ctx.stream()
+        f = "tamx_tile_norms_f64" if self.dtype == torch.float64 else "tamx_tile_norms_f32"
+        call(f, self.ctx.ptr, self.a_plan.ptr, self.A.data_ptr(), self.a_norms.data_ptr(), s)
+        call(f, self.ctx.ptr, self.b_plan.ptr, self.B.data_ptr(), self.b_norms.data_ptr(), s)
+        for norms, sizes, grid in ((self.a_norms, self.a_sizes, (self.Mt, self.Kt)),
+                                   (self.b_norms, self.b_sizes, (self.Kt, self.Nt))):
+            nt = np.ascontiguousarray(grid, np.int64)
+            call("tamx_shape_scale", self.ctx.ptr, norms.data_ptr(), 2, nt.ctypes.data,
+                 self.nbatch, sizes.data_ptr(), 1, ctypes.c_float(self.threshold), None, s)
+        call("tamx_shape_gemm_f32", self.ctx.ptr, self.a_norms.data_ptr(), self.b_norms.data_ptr(),
+             0, 0, self.Mt, self.Nt, self.Kt, self.nbatch, self.k_sizes.data_ptr(),
+             ctypes.c_float(abs(self.alpha)), ctypes.c_float(self.threshold),
+             self.c_norms.data_ptr(), None, s)
+
+    def worklist_step(self):
+        call("tamx_contraction_worklist", self.ctx.ptr, ctypes.byref(self._desc), self.wl.ptr,
+             self.ctx.stream())
+
+    def gemm_step(self):
+        s = self.ctx.stream()
+        cfg = None if self.tile_cfg is None else ctypes.byref(self.tile_cfg)
+        if self.dtype == torch.float64:
+            call("tamx_grouped_gemm_f64_config", self.ctx.ptr, self.wl.ptr, 0, 0,
+                 ctypes.c_double(self.alpha), self.A.data_ptr(), self.B.data_ptr(),
+                 self.C.data_ptr(), cfg, s)
+        else:
+            call("tamx_grouped_gemm_f32_config", self.ctx.ptr, self.wl.ptr, 0, 0,
+                 ctypes.c_float(self.alpha), self.A.data_ptr(), self.B.data_ptr(),
+                 self.C.data_ptr(), cfg, s)
+
+    def run(self):
+        self.shape_step()
+        self.worklist_step()
+        self.gemm_step()
+
+    def counters(self):
+        c = self.wl.counters()
+        if c[_lib.WL_STATUS] != 0:
+            raise _lib.TamxError(c[_lib.WL_STATUS], "tamx_contraction_worklist",
+                                 "capacity exceeded")
+        return {"ntasks": c[_lib.WL_NTASKS], "npairs": c[_lib.WL_NPAIRS],
+                "celems": c[_lib.WL_CELEMS], "flops": c[_lib.WL_FLOPS]}
+
+    def capture(self):
+        """Captures run() into a hipGraph (torch.cuda.CUDAGraph); returns it.
+        Every step is device work with no host synchronisation, no allocation
+        and no memset, so the graph replays the whole contraction."""
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(self.ctx.device)
+        side.wait_stream(torch.cuda.current_stream(self.ctx.device))
+        with torch.cuda.stream(side):
+            self.run()  # warm caches (occupancy queries) outside the capture
+        torch.cuda.current_stream(self.ctx.device).wait_stream(side)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            self.run()
+        return g

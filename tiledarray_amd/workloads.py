@@ -137,3 +137,43 @@ def ladder(o=32, v=256, otiles=2, wlo=48, whi=96, seed=BASE_SEED, device="cuda",
     info = {"o": o, "v": v, "v_tiles": trv.ntiles, "v_widths": trv.extents.tolist(),
             "T_sigma2": 1e-2, "v_order": v_order, "seed": seed}
     return T, V, info
+
+
+def tot_batch(nbatch=1024, grid=8, fill=0.5, lo=16, hi=96, seed=BASE_SEED, dtype=torch.float32,
+              device="cuda", integer=False):
+    """nbatch independent tensor-of-tensor products C_n(i,j;a,b) =
+    A_n(i,k;a,c) * B_n(k,j;c,b) on grid x grid outer cells, each outer cell
+    present with probability `fill`; inner extents a_i, c_k, b_j drawn from
+    [lo, hi] per outer index per instance.  Returns a dict of host metadata
+    and device pools: cells of A stored (a_i x c_k) row-major, of B (c_k x b_j)."""
+    rng = np.random.default_rng(seed)
+    g = grid
+    ad = rng.integers(lo, hi + 1, (nbatch, g))
+    cd = rng.integers(lo, hi + 1, (nbatch, g))
+    bd = rng.integers(lo, hi + 1, (nbatch, g))
+    ma = rng.random((nbatch, g, g)) < fill
+    mb = rng.random((nbatch, g, g)) < fill
+    va = (ad[:, :, None] * cd[:, None, :]).reshape(-1)
+    vb = (cd[:, :, None] * bd[:, None, :]).reshape(-1)
+
+    def offsets(mask, vol):
+        m = mask.reshape(-1)
+        off = np.full(m.size, -1, np.int64)
+        cs = np.concatenate([[0], np.cumsum(np.where(m, vol, 0))])
+        off[m] = cs[:-1][m]
+        return off, int(cs[-1])
+
+    a_off, na = offsets(ma, va)
+    b_off, nb = offsets(mb, vb)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    A = torch.empty(max(1, na), dtype=dtype, device=device)
+    B = torch.empty(max(1, nb), dtype=dtype, device=device)
+    for X in (A, B):
+        if integer:
+            X.copy_(torch.randint(-8, 9, X.shape, generator=gen, device=device))
+        else:
+            X.uniform_(-1.0, 1.0, generator=gen)
+    return {"nbatch": nbatch, "grid": g, "a_dims": ad, "c_dims": cd, "b_dims": bd,
+            "a_off": a_off, "b_off": b_off, "a_vol": va, "b_vol": vb, "A": A, "B": B,
+            "fill_a": float(ma.mean()), "fill_b": float(mb.mean()), "seed": seed}
