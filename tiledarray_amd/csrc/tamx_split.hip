@@ -241,10 +241,12 @@ __device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ (r & 3)); }
 template <int S>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
   constexpr int NG = S;  // levels g = s + t < S
+  // LDS holds only the limb chunks (S * 16 KB, so that two workgroups fit a
+  // CU at S = 5); the grabbed item is passed through the first word of sA
+  // while the buffers are idle between items
   __shared__ bf8_t sA[2][S][SBM * 4];
   __shared__ bf8_t sB[2][S][SBN * 4];
-  __shared__ tamx_gemm_pair s_pairs[64];
-  __shared__ int s_item;
+  int* s_item = reinterpret_cast<int*>(&sA[0][0][0]);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = uni(tid >> 6);
@@ -261,6 +263,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
   const int home = blockIdx.x % kRegions;
   int tried = 0;
   for (;;) {
+    __syncthreads();  // every wave is done with the LDS buffers
     if (tid == 0) {
       int it = -1;
       while (tried < kRegions) {
@@ -274,10 +277,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         }
         ++tried;
       }
-      s_item = it;
+      *s_item = it;
     }
     __syncthreads();
-    const int item = uni(s_item);
+    const int item = uni(*s_item);
     __syncthreads();
     if (item < 0) break;
     int lo;
@@ -317,20 +320,12 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       }
 
     const int pend = t_pb + t_pc;
-    const bool pairs_in_lds = t_pc <= 64;
-    if (pairs_in_lds && t_pc > 0) {
-      const int32_t* src = reinterpret_cast<const int32_t*>(p.pairs + t_pb);
-      int32_t* dst = reinterpret_cast<int32_t*>(s_pairs);
-      constexpr int W = sizeof(tamx_gemm_pair) / sizeof(int32_t);
-      for (int x = tid; x < t_pc * W; x += kThreads) dst[x] = src[x];
-    }
-    __syncthreads();
 
     // load cursor: pair lp, chunk lk0
     int lp = t_pb, lk0 = 0, lkp = 0;
     int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0;
     auto set_pair = [&](int pi) {
-      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
+      const tamx_gemm_pair& pp = p.pairs[pi];
       const int at = uni(pp.a_tile), bt = uni(pp.b_tile);
       const SplitTile* A = p.ta + at;
       const SplitTile* B = p.tb + bt;
@@ -409,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     // compute cursor: pair cp, chunks done since the last fold
     int cp = t_pb, ck0 = 0, ckp = 0, ce = 0, since_fold = 0;
     auto set_compute_pair = [&](int pi) {
-      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
+      const tamx_gemm_pair& pp = p.pairs[pi];
       const SplitTile* A = p.ta + uni(pp.a_tile);
       const SplitTile* B = p.tb + uni(pp.b_tile);
       ckp = uni(A->k_pad);
