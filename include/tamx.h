@@ -347,6 +347,14 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             const tamx_split_plan* b_plan, const void* b_limbs,
                             double* C, void* stream);
 
+/* Two FP32 limbs: FP64 tiles split while staged (hi = fp32(a),
+ * lo = fp32(a - hi)); hi*hi and hi*lo + lo*hi on FP32 MFMA with FP32
+ * accumulation over 16-k chunks, folded into FP64.  FP64 operands in the
+ * FP64 work list's layout; no plan needed. */
+int tamx_grouped_gemm_f32x2(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                            int op_b, double alpha, const double* A,
+                            const double* B, double* C, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,3 +87,32 @@ def test_split_ladder_error_bound(ctx, nslices):
     ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
     err = rel_fro(R.to_dense(), ref)
     assert err <= SPLIT_TOL[nslices], (nslices, err)
+
+
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+def test_f32x2_integer_exact(ctx, op_a, op_b):
+    # small integers split exactly into hi (lo = 0) and their products and
+    # 16-term FP32 sums are exact
+    rng = np.random.default_rng(400 + op_a * 2 + op_b)
+    trm = TiledRange1([0, 40, 130])
+    trk = TiledRange1([0, 17, 64, 150])
+    trn = TiledRange1([0, 96, 97, 180])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True)
+    c = E.Contraction(ctx, "m,n", A, "k,m" if op_a else "m,k", B, "n,k" if op_b else "k,n",
+                      precision="f32x2")
+    c.run()
+    torch.cuda.synchronize()
+    Cref, _, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+def test_f32x2_error_bound(ctx):
+    # FP32 accumulation rounds: ~24 bits, bound 1e-6 relative (Frobenius)
+    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=3)
+    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision="f32x2")
+    c.run()
+    R = c.result()
+    ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
+    err = rel_fro(R.to_dense(), ref)
+    assert err <= 1e-6, err

@@ -129,6 +129,7 @@ SIGNATURES = {
     "tamx_split_plan_sigmas": [_vp, _vp],
     "tamx_split_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_split": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_f32x2": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {
     "tamx_gemm_config_count": ctypes.c_int,

@@ -495,6 +495,273 @@ int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& p
   return TAMX_OK;
 }
 
+
+// ---------------------------------------------------- FP32 two limbs --
+//
+// FP64 tiles are split while they are staged: hi = fp32(a), lo = fp32(a - hi)
+// (a = hi + lo + O(2^-48 a)).  Per 16-k chunk the products hi*hi and
+// hi*lo + lo*hi run on FP32 MFMA (16x16x4) with FP32 accumulation, then are
+// folded into FP64.  FP32 accumulation rounds, so this variant carries about
+// 24 bits; it is reported beside the BF16-limb path, not used for accuracy.
+
+typedef float f4_t2 __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(kThreads) void f32x2_gemm_kernel(
+    const double* __restrict__ A, const double* __restrict__ B, SplitParams p) {
+  constexpr int BM = 64, BN = 64, BK = 16;
+  constexpr int A_LD = BM * BK / kThreads, B_LD = BK * BN / kThreads;
+  constexpr int A_STEP = TA ? kThreads / BM : kThreads / BK;
+  constexpr int B_STEP = TB ? kThreads / BK : kThreads / BN;
+  __shared__ float sAh[2][BK][BM + 1], sAl[2][BK][BM + 1];
+  __shared__ float sBh[2][BK][BN + 1], sBl[2][BK][BN + 1];
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = uni(tid >> 6);
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int a_fast = TA ? tid % BM : tid % BK, a_slow = TA ? tid / BM : tid / BK;
+  const int b_fast = TB ? tid % BK : tid % BN, b_slow = TB ? tid / BK : tid / BN;
+
+  int64_t ntasks64 = p.counters[TAMX_WL_NTASKS];
+  if (ntasks64 < 0 || ntasks64 > p.task_capacity || p.counters[TAMX_WL_STATUS] != 0)
+    ntasks64 = 0;
+  const int32_t ntasks = uni((int32_t)ntasks64);
+  const int32_t total = uni(p.prefix[ntasks > 0 ? ntasks : 0]);
+  const int home = blockIdx.x % kRegions;
+  int tried = 0;
+  for (;;) {
+    if (tid == 0) {
+      int it = -1;
+      while (tried < kRegions) {
+        const int r = (home + tried) % kRegions;
+        const int rb = (int)((int64_t)total * r / kRegions);
+        const int re = (int)((int64_t)total * (r + 1) / kRegions);
+        const int q = rb < re ? atomicAdd(&p.queue[r], 1) : 0;
+        if (rb + q < re) {
+          it = rb + q;
+          break;
+        }
+        ++tried;
+      }
+      s_item = it;
+    }
+    __syncthreads();
+    const int item = uni(s_item);
+    __syncthreads();
+    if (item < 0) break;
+    int lo;
+    if (total <= p.item_capacity) {
+      lo = uni(p.item_task[item]);
+    } else {
+      lo = 0;
+      int hi = ntasks - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uni(p.prefix[mid]) <= item) lo = mid; else hi = mid - 1;
+      }
+    }
+    const tamx_gemm_task* tp = p.tasks + lo;
+    const int64_t t_coff = uni64(tp->c_off);
+    const int t_m = uni(tp->m), t_n = uni(tp->n), t_ldc = uni(tp->ldc);
+    const int t_pb = uni(tp->pair_begin), t_pc = uni(tp->pair_count);
+    const int t_flags = uni(tp->flags);
+    const int local = item - uni(p.prefix[lo]);
+    const int tiles_n = (t_n + BN - 1) / BN;
+    const int r0 = (local / tiles_n) * BM, c0 = (local % tiles_n) * BN;
+    const int mrem = min(BM, t_m - r0), nrem = min(BN, t_n - c0);
+    const bool wave_active = (wr < mrem) && (wc < nrem);
+
+    d4_t acc[2][2];
+    f4_t2 phh[2][2], px[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          acc[i][j][v] = 0.0;
+          phh[i][j][v] = 0.0f;
+          px[i][j][v] = 0.0f;
+        }
+
+    int64_t a_base = 0, b_base = 0;
+    int a_lda = 0, b_ldb = 0, pk = 0;
+    int64_t a_off[A_LD], b_off[B_LD];
+    bool a_ok[A_LD], b_ok[B_LD];
+    auto set_pair = [&](int pi) {
+      const tamx_gemm_pair& pp = p.pairs[pi];
+      a_base = uni64(pp.a_off);
+      b_base = uni64(pp.b_off);
+      pk = uni(pp.k);
+      a_lda = uni(pp.lda);
+      b_ldb = uni(pp.ldb);
+#pragma unroll
+      for (int e = 0; e < A_LD; ++e) {
+        const int slow = a_slow + e * A_STEP;
+        a_ok[e] = TA ? a_fast < mrem : slow < mrem;
+        a_off[e] = TA ? (int64_t)slow * a_lda + r0 + a_fast : (int64_t)(r0 + slow) * a_lda + a_fast;
+      }
+#pragma unroll
+      for (int e = 0; e < B_LD; ++e) {
+        const int slow = b_slow + e * B_STEP;
+        b_ok[e] = TB ? slow < nrem : b_fast < nrem;
+        b_off[e] = TB ? (int64_t)(c0 + slow) * b_ldb + b_fast : (int64_t)slow * b_ldb + c0 + b_fast;
+      }
+    };
+    double ra[A_LD], rb[B_LD];
+    auto load_chunk = [&](int k0) {
+      const int kc = min(BK, pk - k0);
+      const int64_t a_k = TA ? (int64_t)k0 * a_lda : k0;
+      const int64_t b_k = TB ? k0 : (int64_t)k0 * b_ldb;
+#pragma unroll
+      for (int e = 0; e < A_LD; ++e) {
+        const int kk = TA ? a_slow + e * A_STEP : a_fast;
+        const bool ok = a_ok[e] && kk < kc;
+        const double v = A[a_base + (ok ? a_off[e] + a_k : 0)];
+        ra[e] = ok ? v : 0.0;
+      }
+#pragma unroll
+      for (int e = 0; e < B_LD; ++e) {
+        const int kk = TB ? b_fast : b_slow + e * B_STEP;
+        const bool ok = b_ok[e] && kk < kc;
+        const double v = B[b_base + (ok ? b_off[e] + b_k : 0)];
+        rb[e] = ok ? v : 0.0;
+      }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+      for (int e = 0; e < A_LD; ++e) {
+        const float h = (float)ra[e];
+        const float l = (float)(ra[e] - (double)h);
+        const int kk = TA ? a_slow + e * A_STEP : a_fast;
+        const int m = TA ? a_fast : a_slow + e * A_STEP;
+        sAh[buf][kk][m] = h;
+        sAl[buf][kk][m] = l;
+      }
+#pragma unroll
+      for (int e = 0; e < B_LD; ++e) {
+        const float h = (float)rb[e];
+        const float l = (float)(rb[e] - (double)h);
+        const int kk = TB ? b_fast : b_slow + e * B_STEP;
+        const int n = TB ? b_slow + e * B_STEP : b_fast;
+        sBh[buf][kk][n] = h;
+        sBl[buf][kk][n] = l;
+      }
+    };
+    auto compute_chunk = [&](int buf) {
+      if (!wave_active) return;
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          ah[i] = sAh[buf][ks + lk][wr + i * 16 + lrow];
+          al[i] = sAl[buf][ks + lk][wr + i * 16 + lrow];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          bh[j] = sBh[buf][ks + lk][wc + j * 16 + lrow];
+          bl[j] = sBl[buf][ks + lk][wc + j * 16 + lrow];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            phh[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[i], bh[j], phh[i][j], 0, 0, 0);
+            px[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[i], bl[j], px[i][j], 0, 0, 0);
+            px[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[i], bh[j], px[i][j], 0, 0, 0);
+          }
+      }
+      // fold this chunk into FP64
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            acc[i][j][v] += (double)phh[i][j][v] + (double)px[i][j][v];
+            phh[i][j][v] = 0.0f;
+            px[i][j][v] = 0.0f;
+          }
+    };
+
+    int pi = t_pb;
+    const int pend = t_pb + t_pc;
+    if (pi < pend) {
+      set_pair(pi);
+      int k0 = 0;
+      load_chunk(k0);
+      store_chunk(0);
+      __syncthreads();
+      int buf = 0;
+      k0 += BK;
+      if (k0 >= pk) {
+        ++pi;
+        k0 = 0;
+        if (pi < pend) set_pair(pi);
+      }
+      for (;;) {
+        const bool has_next = pi < pend;
+        if (has_next) load_chunk(k0);
+        compute_chunk(buf);
+        if (!has_next) break;
+        store_chunk(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+        k0 += BK;
+        if (k0 >= pk) {
+          ++pi;
+          k0 = 0;
+          if (pi < pend) set_pair(pi);
+        }
+      }
+    }
+    const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
+    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
+    double* __restrict__ c = p.C + t_coff;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = wr + i * 16 + lk * 4 + v;  // f32 16x16 MFMA layout
+          const int col = wc + j * 16 + lrow;
+          if (row < mrem && col < nrem) {
+            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
+                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
+            double val = p.alpha * acc[i][j][v];
+            if (beta1) val += c[idx];
+            c[idx] = val;
+          }
+        }
+  }
+  if (tid == 0) {
+    __threadfence();
+    const int done = atomicAdd(&p.queue[kRegions], 1);
+    if (done == (int)gridDim.x - 1) {
+      for (int r = 0; r <= kRegions; ++r) atomicExch(&p.queue[r], 0);
+    }
+  }
+}
+
+template <bool TA, bool TB>
+int launch_f32x2(tamx_context* ctx, tamx_worklist* wl, const double* A, const double* B,
+                 const SplitParams& prm, hipStream_t s) {
+  auto kern = f32x2_gemm_kernel<TA, TB>;
+  int occ = 0;
+  TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
+  if (occ < 1) occ = 1;
+  split_plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, wl->item_prefix,
+                                             wl->item_task, wl->task_capacity,
+                                             wl->item_capacity);
+  TAMX_LAUNCH_CHECK();
+  kern<<<ctx->num_cus * occ, kThreads, 0, s>>>(A, B, prm);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -654,6 +921,23 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
     case 5: return launch_split_gemm<5>(ctx, wl, prm, s);
     default: return launch_split_gemm<6>(ctx, wl, prm, s);
   }
+}
+
+int tamx_grouped_gemm_f32x2(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
+                            double alpha, const double* A, const double* B, double* C,
+                            void* stream) {
+  TAMX_REQUIRE(wl != nullptr, "null work list");
+  TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be TAMX_NO_TRANS or TAMX_TRANS");
+  TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be TAMX_NO_TRANS or TAMX_TRANS");
+  if (int rc = tamx::activate(ctx)) return rc;
+  SplitParams prm{nullptr, nullptr, nullptr, nullptr, C, wl->tasks, wl->pairs, wl->counters,
+                  wl->item_prefix, wl->item_task, wl->queue, wl->task_capacity,
+                  wl->item_capacity, alpha};
+  hipStream_t s = tamx::as_stream(stream);
+  if (!op_a && !op_b) return launch_f32x2<false, false>(ctx, wl, A, B, prm, s);
+  if (!op_a && op_b) return launch_f32x2<false, true>(ctx, wl, A, B, prm, s);
+  if (op_a && !op_b) return launch_f32x2<true, false>(ctx, wl, A, B, prm, s);
+  return launch_f32x2<true, true>(ctx, wl, A, B, prm, s);
 }
 
 }  // extern "C"

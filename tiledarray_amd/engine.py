@@ -320,7 +320,7 @@ class Contraction:
             self.nslices = int(precision[5:])
             self.a_split = self._split_plan(self.A, False, self.op_a)
             self.b_split = self._split_plan(self.B, True, self.op_b)
-        elif precision not in ("f64", "f32"):
+        elif precision not in ("f64", "f32", "f32x2"):
             raise ValueError(f"unknown precision {precision!r}")
 
     def _split_plan(self, X, is_right, op):
@@ -423,6 +423,9 @@ class Contraction:
         elif self.precision == "f32":
             call("tamx_grouped_gemm_f32_config", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
                  ctypes.c_float(self.alpha), A, B, C, cfg, s)
+        elif self.precision == "f32x2":
+            call("tamx_grouped_gemm_f32x2", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
+                 ctypes.c_double(self.alpha), A, B, C, s)
         else:
             call("tamx_grouped_gemm_split", self.ctx.ptr, self.wl.ptr, ctypes.c_double(self.alpha),
                  self.a_split.ptr, self.a_split.limbs.data_ptr(), self.b_split.ptr,
