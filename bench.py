@@ -27,9 +27,12 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
-# HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE passes (KiB per dispatch x 1024), profiles/ holds the summaries
-TRAFFIC_BYTES = {("blocksparse", "f64"): 3044494 * 2048 + 925737 * 1024}
+# HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes, each scaled by what the counter reads for a plain copy
+# of 1 GiB (524302 FETCH_SIZE units read, 1048576 WRITE_SIZE units written:
+# 2 KiB and 1 KiB per unit).  profiles/ holds the summaries.  Keyed by
+# workload, precision and macro tile.
+TRAFFIC_BYTES = {("blocksparse", "f64", (64, 64, 16)): 3044494 * 2048 + 925737 * 1024}
 
 
 def parse():
@@ -46,14 +49,35 @@ def parse():
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only copy 1 GiB three times (scales FETCH_SIZE/WRITE_SIZE to bytes)")
     p.add_argument("--n", type=int, default=None, help="matrix extent (default per workload)")
-    p.add_argument("--tile", default=None, help="macro tile BMxBN[xBK[xWGS]] for the grouped GEMM")
+    p.add_argument("--tile", default="auto",
+                   help="macro tile BMxBN[xBK[xWGS]] of the grouped GEMM, or 'auto' (autotuner)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--precision", default="f64", help="f64, f32x2 or bf16xS (S BF16 limbs)")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision runs")
-    p.add_argument("--split", default="bf16x3,bf16x5,f32x2",
+    p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,f32x2",
                    help="split precisions measured beside the FP64 run")
     return p.parse_args()
+
+
+def fixed_tile(args):
+    """The --tile argument as a tuple, or None for 'auto'."""
+    if args.tile in (None, "auto"):
+        return None
+    return tuple(int(x) for x in args.tile.split("x"))
+
+
+def autotune(args, c, step_fn, reps=3):
+    """With --tile auto: one untimed step builds the work list, then the
+    autotuner times every compiled tile shape on it and the contraction
+    keeps the fastest.  Returns the tile to report."""
+    import torch
+    if args.tile != "auto":
+        return list(fixed_tile(args)) if args.tile else None
+    step_fn(None)
+    torch.cuda.synchronize()
+    tile, _ = c.tune(reps=reps)
+    return list(tile[:3])
 
 
 def time_steps(c, step_fn, steps, warmup):
@@ -115,7 +139,7 @@ def main():
         A, B, info = W.dense_gemm(n=n, tile=128)
         workload = f"dense FP64 C(m,n)=A(m,k)*B(k,n), M=N=K={n}, 128-wide tiles, U[-1,1]"
         sparse = False
-    tile = tuple(int(x) for x in args.tile.split("x")) if args.tile else None
+    tile = fixed_tile(args)
 
     sizes = {}
     if sparse:
@@ -150,6 +174,7 @@ def main():
 
     c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, tile=tile,
                       precision=args.precision)
+    used_tile = autotune(args, c, make_step(c)) if args.precision in ("f64", "f32") else None
     ms, gemm_ms = time_steps(c, make_step(c), args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
@@ -176,18 +201,21 @@ def main():
             "result_tiles": cnt["ntasks"],
             "tile_pairs": cnt["npairs"],
             "flops_per_step": flops,
-            "macro_tile": list(tile) if tile else [64, 64, 16],
+            "macro_tile": used_tile if used_tile else ([64, 64, 32] if args.precision != "f64"
+                                                       else [64, 64, 16]),
+            "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
             "seed": info["seed"],
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "grouped_gemm_kernel<double,64,64,16>" if args.precision == "f64"
-            else "split_gemm_kernel",
+            "kernel": (f"grouped_gemm_kernel<double,{','.join(map(str, used_tile or [64, 64, 16]))}>"
+                       if args.precision == "f64" else "split_gemm_kernel"),
             "achieved": round(achieved, 3),
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": TRAFFIC_BYTES.get((args.workload, args.precision)),
+            "traffic": TRAFFIC_BYTES.get((args.workload, args.precision,
+                                          tuple(used_tile or [64, 64, 16]))),
             "avg_launch_ms": round(gemm_ms, 4),
         },
     }
@@ -249,7 +277,7 @@ def bench_ladder(args):
     E.make_sparse(ctx, V, thr)
     sizes = {id(X): E._sizes_vector(X.trange, X.pool.device) for X in (T, V)}
     c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", threshold=thr,
-                      precision=args.precision)
+                      precision=args.precision, tile=fixed_tile(args))
     # the permuted V's shape follows the source's; refresh it each step
     perm_norms = c.B.shape_norms
 
@@ -274,6 +302,7 @@ def bench_ladder(args):
         else:
             c.gemm_step()
 
+    used_tile = autotune(args, c, step, reps=1) if args.precision == "f64" else None
     ms, gemm_ms = time_steps(c, step, args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
@@ -283,6 +312,8 @@ def bench_ladder(args):
                     f"ladder R(a,b,i,j)=T(c,d,i,j)*V(a,c,b,d), o={args.o}, v={args.v}, "
                     f"v tiles U[48,96], T~N(0,1e-2), V~N(0,1)exp(-|a-c|/64), threshold 1e-10",
                     {"v_tiles": info["v_tiles"], "result_tiles": cnt["ntasks"],
+                     "macro_tile": used_tile or ([64, 64, 16] if args.precision == "f64"
+                                                 else [64, 64, 32]),
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
                      "precision": args.precision, "seed": info["seed"]},
                     {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),
@@ -380,8 +411,12 @@ def bench_tot(args):
     g = w["grid"]
     c = E.BatchedContraction(ctx, args.nbatch, g, g, g, w["a_dims"], w["b_dims"], w["c_dims"],
                              w["A"], w["a_off"], w["B"], w["b_off"],
-                             tile=(32, 32, 16) if args.tile is None else
-                             tuple(int(x) for x in args.tile.split("x")))
+                             tile=fixed_tile(args))
+    used_tile = list(fixed_tile(args) or (64, 64, 16))[:3]
+    if args.tile == "auto":
+        c.run()
+        torch.cuda.synchronize()
+        used_tile = list(c.tune(reps=3)[0][:3])
     graph = c.capture()
     for _ in range(args.warmup):
         graph.replay()
@@ -408,7 +443,9 @@ def bench_tot(args):
                     f"{args.nbatch} tensor-of-tensor products C_n(i,j;a,b)=A_n(i,k;a,c)*B_n(k,j;c,b),"
                     f" 8x8 outer grids 50% filled, inner dims U[16,96], hipGraph replay",
                     {"result_cells": cnt["ntasks"], "cell_pairs": cnt["npairs"],
-                     "flops_per_step": flops, "seed": w["seed"]},
+                     "flops_per_step": flops, "macro_tile": used_tile,
+                     "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
+                     "seed": w["seed"]},
                     {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,

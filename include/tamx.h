@@ -313,8 +313,10 @@ int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
 
 /* Splitting of FP64 operand tiles into nslices BF16 limbs with one
  * power-of-two scale per tile (sigma = exponent of the tile's max |a| plus
- * one): a = 2^sigma * sum_s l_s 2^-8(s+1) + r, |r| <= 2^(sigma-8*nslices-1),
- * every l_s an integer held exactly in BF16.  The limbs are laid out for the
+ * one).  The first limb takes 8 bits of the tile-scaled value, every later
+ * limb 9:  a = 2^(sigma-8) * sum_s L_s + r,  |r| <= 2^(sigma - 9*nslices),
+ * with L_s = l_s 2^-9s, l_s an integer, |l_s| <= 256, so every L_s is held
+ * exactly in BF16.  The limbs are laid out for the
  * MFMA: the logical op(A) tile as m x K and op(B) as n x K, K contiguous,
  * rows padded to 16 and K to 32 with zeros.  No reference counterpart: the
  * reference multiplies in FP64 only (math/blas.h:171-177); this is the
@@ -338,8 +340,9 @@ int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma);
 int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan,
                    const double* pool, void* limbs, void* stream);
 /* Grouped GEMM on split operands: per pair the limb products with
- * s + t < nslices on BF16 MFMA with FP32 accumulation, folded into FP64
- * (cross-limb sums, the 2^(sigma_A + sigma_B) tile scaling and the C
+ * s + t < nslices on BF16 MFMA with FP32 accumulation (the s + t = 0 and
+ * s + t = 1 sums exact, the lower levels together), folded into FP64 every
+ * 128 k (cross-limb sums, the 2^(sigma_A + sigma_B) tile scaling and the C
  * accumulation in FP64).  Same work list and beta/transpose semantics as
  * tamx_grouped_gemm_f64. */
 int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
@@ -354,6 +357,48 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
 int tamx_grouped_gemm_f32x2(tamx_context* ctx, tamx_worklist* wl, int op_a,
                             int op_b, double alpha, const double* A,
                             const double* B, double* C, void* stream);
+
+/* ------------------------------------------------------ autotuning --- */
+
+/* Times every compiled tile configuration (tamx_gemm_config_get) of the
+ * FP64 (precision TAMX_PREC_F64) or FP32 (TAMX_PREC_F32) grouped GEMM on a
+ * built work list and returns the fastest in *best (and its mean launch
+ * time, item planning included, in *best_ms).  The caller's C is not
+ * modified: the trial launches write into a temporary buffer, seeded from C
+ * when a task accumulates into it (TAMX_TASK_BETA1).  Each configuration
+ * runs once untimed (unless TAMX_TUNE_NO_WARMUP) and then `reps` times.
+ * Synchronises `stream` and allocates; not for use under graph capture.
+ * Results are cached per device and shape class: precision, ops, and the
+ * half-octave buckets of the mean task m and n, the mean pair k and the
+ * task count.  A cached class returns without timing (*best_ms = the time
+ * measured then) unless flags has TAMX_TUNE_FORCE.
+ * No reference counterpart: the reference calls a vendor BLAS with its own
+ * fixed blocking per tile pair (math/blas.h:171-177). */
+#define TAMX_TUNE_FORCE 1
+#define TAMX_TUNE_NO_WARMUP 2
+int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl,
+                               int precision, int op_a, int op_b,
+                               const void* A, const void* B, const void* C,
+                               int reps, int flags, tamx_gemm_config* best,
+                               float* best_ms, void* stream);
+/* Drops every cached tuning result of the process. */
+int tamx_autotune_clear(void);
+
+/* --------------------------------------------------------- graphs --- */
+
+/* Capture of a launch sequence on a stream into a replayable graph.  Every
+ * entry point above that neither synchronises nor allocates may be called
+ * between tamx_graph_begin and tamx_graph_end (norms, shape scaling and
+ * estimate, work-list build, permutes, limb split, grouped GEMMs); the
+ * graph replays them with the arguments of capture time, reading the
+ * current contents of the buffers they name.  `stream` must be a created
+ * stream (not the null stream).  No reference counterpart: the reference
+ * launches per tile pair (contract_reduce.h:446-451). */
+typedef struct tamx_graph tamx_graph;
+int tamx_graph_begin(tamx_context* ctx, void* stream);
+int tamx_graph_end(tamx_context* ctx, void* stream, tamx_graph** graph);
+int tamx_graph_launch(tamx_graph* graph, void* stream);
+int tamx_graph_destroy(tamx_graph* graph);
 
 #ifdef __cplusplus
 }

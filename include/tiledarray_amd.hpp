@@ -122,6 +122,34 @@ class World {
   hipStream_t stream_ = nullptr;
 };
 
+// A launch sequence captured on the world's stream (tamx_graph_begin/_end)
+// and replayed by launch().  Only calls that neither synchronise nor
+// allocate may run inside the capture (norms, shape estimate, work list,
+// permutes, limb split, grouped GEMMs on buffers that outlive the graph).
+class Graph {
+ public:
+  template <typename F>
+  Graph(World& w, F&& fn) : w_(&w) {
+    check(tamx_graph_begin(w.ctx(), w.stream()), "tamx_graph_begin");
+    try {
+      fn();
+    } catch (...) {
+      tamx_graph* g = nullptr;
+      if (tamx_graph_end(w.ctx(), w.stream(), &g) == TAMX_OK) tamx_graph_destroy(g);
+      throw;
+    }
+    check(tamx_graph_end(w.ctx(), w.stream(), &g_), "tamx_graph_end");
+  }
+  ~Graph() { tamx_graph_destroy(g_); }
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+  void launch() const { check(tamx_graph_launch(g_, w_->stream()), "tamx_graph_launch"); }
+
+ private:
+  World* w_;
+  tamx_graph* g_ = nullptr;
+};
+
 // Default SparseShape<float> threshold (sparse_shape.h:1941-1942).
 inline float default_threshold() { return std::numeric_limits<float>::epsilon(); }
 
@@ -225,6 +253,7 @@ struct ContractOptions {
   float pair_threshold = 0.0f;            // per-pair filter (off, as in TA)
   Precision precision = Precision::FP64;
   int nslices = 5;                        // BF16 limbs for BF16_LIMBS
+  bool autotune = false;                  // pick the tile shape by timing (FP64/FP32)
 };
 
 template <typename T>
@@ -583,18 +612,26 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
       sb(nullptr, tamx_split_plan_destroy);
   DeviceBuffer<uint16_t> la_limbs, lb_limbs;
   if (opt.precision == Precision::FP64 || opt.precision == Precision::FP32) {
-    if (std::is_same<T, double>::value)
-      check(tamx_grouped_gemm_f64(w.ctx(), wl, plan.op_a, plan.op_b, factor,
-                                  reinterpret_cast<const double*>(A->pool()),
-                                  reinterpret_cast<const double*>(B->pool()),
-                                  reinterpret_cast<double*>(C.get()), w.stream()),
-            "tamx_grouped_gemm_f64");
+    const bool f64 = std::is_same<T, double>::value;
+    tamx_gemm_config cfg{};
+    check(tamx_gemm_config_get(0, &cfg), "tamx_gemm_config_get");
+    if (opt.autotune)
+      check(tamx_autotune_grouped_gemm(w.ctx(), wl, f64 ? TAMX_PREC_F64 : TAMX_PREC_F32,
+                                       plan.op_a, plan.op_b, A->pool(), B->pool(), C.get(),
+                                       3, 0, &cfg, nullptr, w.stream()),
+            "tamx_autotune_grouped_gemm");
+    if (f64)
+      check(tamx_grouped_gemm_f64_config(w.ctx(), wl, plan.op_a, plan.op_b, factor,
+                                         reinterpret_cast<const double*>(A->pool()),
+                                         reinterpret_cast<const double*>(B->pool()),
+                                         reinterpret_cast<double*>(C.get()), &cfg, w.stream()),
+            "tamx_grouped_gemm_f64_config");
     else
-      check(tamx_grouped_gemm_f32(w.ctx(), wl, plan.op_a, plan.op_b, (float)factor,
-                                  reinterpret_cast<const float*>(A->pool()),
-                                  reinterpret_cast<const float*>(B->pool()),
-                                  reinterpret_cast<float*>(C.get()), w.stream()),
-            "tamx_grouped_gemm_f32");
+      check(tamx_grouped_gemm_f32_config(w.ctx(), wl, plan.op_a, plan.op_b, (float)factor,
+                                         reinterpret_cast<const float*>(A->pool()),
+                                         reinterpret_cast<const float*>(B->pool()),
+                                         reinterpret_cast<float*>(C.get()), &cfg, w.stream()),
+            "tamx_grouped_gemm_f32_config");
   } else {
     if (!std::is_same<T, double>::value)
       throw Exception(TAMX_ERR_UNSUPPORTED, "split precision takes FP64 operands");

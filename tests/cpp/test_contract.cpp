@@ -230,6 +230,60 @@ int main() {
     CHECK(err <= 1e-10, "relative error %g", err);
   });
 
+  run("autotuned tile shape, block-sparse", [&] {
+    TiledRange1 tb{0, 37, 90, 150, 151, 260, 333};
+    auto A = make_array<double>(world, TiledRange{tb, tb}, rng, true, 0.7);
+    auto B = make_array<double>(world, TiledRange{tb, tb}, rng, true, 0.7);
+    DistArray<double> C(world, TiledRange{tb, tb});
+    ContractOptions opt;
+    opt.autotune = true;
+    C("i,j") = (A("i,k") * B("k,j")).with(opt);
+    std::vector<int64_t> er;
+    auto ref = einsum(dense(A), "i,k", extents(A), dense(B), "k,j", extents(B), "i,j", er);
+    CHECK(dense(C) == ref, "autotuned result differs");
+  });
+
+  run("graph capture and replay of a tile permutation", [&] {
+    auto V = make_array<double>(world, TiledRange{tv, to, tv}, rng, true);
+    const int64_t n = V.pool_elems();
+    DeviceBuffer<double> dst(n);
+    const int32_t perm[3] = {2, 0, 1};
+    std::vector<int64_t> ext, src_off, dst_off;
+    for (size_t t = 0; t < V.trange().ntiles(); ++t) {
+      auto e = V.trange().tile_extents(t);
+      ext.insert(ext.end(), e.begin(), e.end());
+      src_off.push_back(V.offsets()[t]);
+      dst_off.push_back(V.offsets()[t]);
+    }
+    tamx_permute_plan* plan = nullptr;
+    check(tamx_permute_plan_create(world.ctx(), 3, perm, (int64_t)V.trange().ntiles(), ext.data(),
+                                   src_off.data(), dst_off.data(), &plan),
+          "tamx_permute_plan_create");
+    check(tamx_permute_f64(world.ctx(), plan, V.pool(), dst.get(), world.stream()),
+          "tamx_permute_f64");
+    std::vector<double> eager(n), replay(n);
+    hip_check(hipMemcpyAsync(eager.data(), dst.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                             static_cast<hipStream_t>(world.stream())), "copy");
+    world.fence();
+    hip_check(hipMemset(dst.get(), 0, n * sizeof(double)), "clear");
+    {
+      Graph g(world, [&] {
+        check(tamx_permute_f64(world.ctx(), plan, V.pool(), dst.get(), world.stream()),
+              "tamx_permute_f64");
+      });
+      for (int rep = 0; rep < 3; ++rep) {
+        hip_check(hipMemset(dst.get(), 0, n * sizeof(double)), "clear");
+        g.launch();
+        hip_check(hipMemcpyAsync(replay.data(), dst.get(), n * sizeof(double),
+                                 hipMemcpyDeviceToHost, static_cast<hipStream_t>(world.stream())),
+                  "copy");
+        world.fence();
+        CHECK(replay == eager, "replay %d differs from the eager permute", rep);
+      }
+    }
+    tamx_permute_plan_destroy(plan);
+  });
+
   run("float tiles", [&] {
     auto A = make_array<float>(world, TiledRange{ti, tk}, rng, true);
     auto B = make_array<float>(world, TiledRange{tk, tj}, rng, true);

@@ -21,10 +21,12 @@ from test_gpu_contract import c_off_mask, matrix_ops_arrays, oracle_contract, re
 
 pytestmark = pytest.mark.gpu
 
-# relative Frobenius error bounds per limb count on U[-1,1] / Gaussian data
-# (a limb carries ~8 bits; the bound leaves ~30x headroom over what the
-# expansion's truncation gives)
-SPLIT_TOL = {2: 3e-3, 3: 3e-5, 4: 1e-7, 5: 1e-10, 6: 1e-12}
+# relative Frobenius error bounds per limb count (the first limb carries 8
+# bits, every later one 9; the bounds leave ~30x headroom over a numpy
+# simulation of the same expansion: U[-1,1] tiles 1.5e-8 / 3.3e-11 / 4.9e-13
+# at 3 / 4 / 5 limbs, ladder tiles 3.6e-7 / 1.2e-9 / 6.0e-12)
+SPLIT_TOL = {2: 1e-3, 3: 1e-6, 4: 1e-9, 5: 1e-11, 6: 1e-12}
+LADDER_TOL = {3: 1e-5, 4: 3e-8, 5: 2e-10}
 
 
 @pytest.mark.parametrize("nslices", [2, 3, 5, 6])
@@ -78,7 +80,7 @@ def test_split_blocksparse_error_bound(ctx, nslices):
     assert err <= SPLIT_TOL[nslices], (nslices, err)
 
 
-@pytest.mark.parametrize("nslices", [3, 5])
+@pytest.mark.parametrize("nslices", [3, 4, 5])
 def test_split_ladder_error_bound(ctx, nslices):
     T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=3)
     c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision=f"bf16x{nslices}")
@@ -86,7 +88,66 @@ def test_split_ladder_error_bound(ctx, nslices):
     R = c.result()
     ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
     err = rel_fro(R.to_dense(), ref)
-    assert err <= SPLIT_TOL[nslices], (nslices, err)
+    assert err <= LADDER_TOL[nslices], (nslices, err)
+
+
+@pytest.mark.parametrize("nslices", [2, 3, 4, 5])
+@pytest.mark.parametrize("is_right,op", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_split_limbs_closed_form(ctx, nslices, is_right, op):
+    # the limbs of one tile against the expansion's definition:
+    #   L_s 2^(9s) is an integer l_s, |l_0| <= 256, |l_s| <= 256,
+    #   |a - 2^(sigma-8) sum_s L_s| <= 2^(sigma - 9S), padding is zero
+    import ctypes
+    from tiledarray_amd import _lib
+    rows, cols = 37, 45  # stored (row-major) tile
+    rng = np.random.default_rng(700 + nslices)
+    a = rng.uniform(-1, 1, (rows, cols)) * 3.7 * 2.0 ** rng.integers(-3, 4, (rows, cols))
+    pool = torch.as_tensor(a.reshape(-1), device="cuda")
+    plan = ctypes.c_void_p()
+    off = np.zeros(1, np.int64)
+    r32 = np.array([rows], np.int32)
+    c32 = np.array([cols], np.int32)
+    E.call("tamx_split_plan_create", ctx.ptr, nslices, is_right, op, 1, off.ctypes.data,
+           r32.ctypes.data, c32.ctypes.data, ctypes.byref(plan))
+    try:
+        n = _lib.lib().tamx_split_plan_limb_elems(plan)
+        limbs = torch.zeros(n, dtype=torch.bfloat16, device="cuda")
+        E.call("tamx_split_f64", ctx.ptr, plan, pool.data_ptr(), limbs.data_ptr(), ctx.stream())
+        torch.cuda.synchronize()
+        sig = np.zeros(1, np.int32)
+        E.call("tamx_split_plan_sigmas", plan, sig.ctypes.data)
+    finally:
+        _lib.lib().tamx_split_plan_destroy(plan)
+    sigma = int(sig[0])
+    assert 2.0 ** (sigma - 1) <= np.abs(a).max() < 2.0 ** sigma
+    # logical limb layout: op(A) as m x K (A) or op(B)^T as n x K (B)
+    trans = (op == 1) if not is_right else (op == 0)
+    logical = a.T if trans else a
+    m, k = logical.shape
+    mp, kp = -(-m // 16) * 16, -(-k // 32) * 32
+    L = limbs.float().cpu().numpy().astype(np.float64)[:nslices * mp * kp]
+    L = L.reshape(nslices, mp, kp)
+    for s_ in range(nslices):
+        l = L[s_] * 2.0 ** (9 * s_)
+        assert np.array_equal(l, np.rint(l)), s_
+        assert np.abs(l).max() <= 256, s_
+    assert not L[:, m:, :].any() and not L[:, :, k:].any()
+    rec = 2.0 ** (sigma - 8) * L[:, :m, :k].sum(axis=0)
+    assert np.abs(logical - rec).max() <= 2.0 ** (sigma - 9 * nslices)
+
+
+def test_split_error_drops_nine_bits_per_limb(ctx):
+    # one more limb divides the error by about 2^9 (between 2^7 and 2^11)
+    A, B, _ = W.dense_gemm(n=384, tile=128)
+    errs = {}
+    ref = A.to_dense() @ B.to_dense()
+    for S in (2, 3, 4):
+        c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision=f"bf16x{S}")
+        c.run()
+        errs[S] = rel_fro(c.result().to_dense(), ref)
+    for S in (2, 3):
+        ratio = errs[S] / errs[S + 1]
+        assert 2.0 ** 7 <= ratio <= 2.0 ** 11, (S, errs)
 
 
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))

@@ -11,11 +11,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libtamx.so")
 
 TAMX_OK = 0
+TAMX_ERR_BAD_ARG, TAMX_ERR_OOM, TAMX_ERR_UNSUPPORTED, TAMX_ERR_DEVICE, TAMX_ERR_CAPACITY = 1, 2, 3, 4, 5
 TAMX_NO_TRANS, TAMX_TRANS = 0, 1
 TAMX_TASK_BETA1, TAMX_TASK_STORE_TRANS = 1, 2
 WL_NTASKS, WL_NPAIRS, WL_CELEMS, WL_STATUS, WL_FLOPS, WL_NCOUNTERS = 0, 1, 2, 3, 4, 8
 PLAN_MAX_RANK, PLAN_NAME_LEN = 8, 64
 PREC_F64, PREC_F32, PREC_BF16X3, PREC_BF16XS, PREC_F32X2 = 0, 1, 2, 3, 4
+TUNE_FORCE, TUNE_NO_WARMUP = 1, 2
 
 
 class TamxError(RuntimeError):
@@ -130,6 +132,12 @@ SIGNATURES = {
     "tamx_split_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_split": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32x2": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp],
+    "tamx_autotune_grouped_gemm": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp],
+    "tamx_autotune_clear": [],
+    "tamx_graph_begin": [_vp, _vp],
+    "tamx_graph_end": [_vp, _vp, _vp],
+    "tamx_graph_launch": [_vp, _vp],
+    "tamx_graph_destroy": [_vp],
 }
 _RESTYPES = {
     "tamx_gemm_config_count": ctypes.c_int,

@@ -107,15 +107,14 @@ int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
   alloc(&wl->tile_npairs, sizeof(int32_t) * gc);
   alloc(&wl->tile_volume, sizeof(int64_t) * gc);
   alloc(&wl->tile_flops, sizeof(int64_t) * gc);
-  alloc(&wl->tile_task, sizeof(int32_t) * gc);
-  alloc(&wl->tile_pair_begin, sizeof(int32_t) * gc);
-  alloc(&wl->tile_c_begin, sizeof(int64_t) * gc);
   alloc(&wl->item_prefix, sizeof(int32_t) * (tc + 1));
   // room for 16 macro tiles per task; larger lists fall back to a binary
   // search over item_prefix inside the GEMM kernel
   wl->item_capacity = std::min<int64_t>(tc * 16, (int64_t(1) << 31) - 1);
   alloc(&wl->item_task, sizeof(int32_t) * wl->item_capacity);
   alloc(&wl->queue, sizeof(int32_t) * 16);
+  const int64_t nparts = std::max<int64_t>(tamx::kScanBlocks, gc / 256 + 1) + 1;
+  alloc(reinterpret_cast<char**>(&wl->scan_partials), 32 * nparts);
   if (e == hipSuccess) e = hipMemset(wl->counters, 0, sizeof(int64_t) * TAMX_WL_NCOUNTERS);
   if (e == hipSuccess) e = hipMemset(wl->queue, 0, sizeof(int32_t) * 16);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -132,8 +131,7 @@ int tamx_worklist_destroy(tamx_worklist* wl) {
   if (!wl) return TAMX_OK;
   void* ptrs[] = {wl->tasks,       wl->pairs,          wl->counters,
                   wl->tile_npairs, wl->tile_volume,    wl->tile_flops,
-                  wl->tile_task,   wl->tile_pair_begin, wl->tile_c_begin,
-                  wl->item_prefix, wl->item_task, wl->queue};
+                  wl->item_prefix, wl->item_task, wl->queue, wl->scan_partials};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete wl;

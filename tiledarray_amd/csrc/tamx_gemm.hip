@@ -62,34 +62,60 @@ struct GemmParams {
 };
 
 constexpr int kThreads = 256;
-constexpr int kRegions = 8;
-constexpr int kPairWin = 64;  // pair descriptors staged in LDS at a time  // XCDs of an MI355X; the queue holds kRegions + 1 counters
+constexpr int kRegions = 8;   // XCDs of an MI355X; the queue holds kRegions + 1 counters
+constexpr int kPairWin = 64;  // pair descriptors staged in LDS at a time
 
-// prefix[t] = number of BM x BN macro tiles of tasks 0..t-1, and the
-// item -> task map when it fits in item_capacity entries.
-__global__ __launch_bounds__(1024) void plan_items_kernel(
+// Item planner: prefix[t] = number of BM x BN macro tiles of tasks 0..t-1,
+// and the item -> task map when it fits in item_capacity entries.  Three
+// launches (chunk sums over kScanBlocks workgroups, one-workgroup scan of the
+// sums, chunk rescans), each walking the task list once.
+__device__ inline int64_t planned_tasks(const int64_t* counters, int64_t capacity) {
+  const int64_t n = counters[TAMX_WL_NTASKS];
+  return (n < 0 || n > capacity || counters[TAMX_WL_STATUS] != 0) ? 0 : n;
+}
+__device__ inline int32_t task_items(const tamx_gemm_task* tasks, int64_t i, int bm,
+                                     int bn) {
+  const tamx_gemm_task t = tasks[i];
+  return ((t.m + bm - 1) / bm) * ((t.n + bn - 1) / bn);
+}
+
+constexpr int kPlanThreads = 256;
+
+__global__ __launch_bounds__(kPlanThreads) void plan_items_sum_kernel(
     const tamx_gemm_task* __restrict__ tasks, const int64_t* __restrict__ counters,
-    int bm, int bn, int32_t* __restrict__ prefix, int32_t* __restrict__ item_task,
-    int64_t capacity, int64_t item_capacity) {
+    int bm, int bn, int64_t capacity, int32_t* __restrict__ partial) {
+  __shared__ int32_t red[kPlanThreads];
+  const int64_t n = planned_tasks(counters, capacity);
+  tamx::scan_phase1<kPlanThreads, int32_t>(
+      n, red, [&](int64_t i) { return task_items(tasks, i, bm, bn); }, partial);
+}
+
+__global__ __launch_bounds__(1024) void plan_items_offsets_kernel(int32_t* partial) {
   __shared__ int32_t part[1024];
-  int64_t n = counters[TAMX_WL_NTASKS];
-  if (n < 0 || n > capacity || counters[TAMX_WL_STATUS] != 0) n = 0;
-  auto count = [&](int64_t i) -> int32_t {
-    const tamx_gemm_task t = tasks[i];
-    return ((t.m + bm - 1) / bm) * ((t.n + bn - 1) / bn);
-  };
-  const int32_t total = tamx::block_exclusive_scan<1024, int32_t>(
-      n, part, count, [&](int64_t i, int32_t v) { prefix[i] = v; });
-  if (threadIdx.x == 0) prefix[n] = total;
-  if (total <= item_capacity) {
-    // each thread fills the items of a contiguous range of tasks
-    const int64_t seg = (n + 1023) / 1024;
-    const int64_t b = min<int64_t>(n, threadIdx.x * seg), e = min<int64_t>(n, b + seg);
-    for (int64_t i = b; i < e; ++i) {
-      const int32_t q0 = prefix[i], q1 = q0 + count(i);
-      for (int32_t q = q0; q < q1; ++q) item_task[q] = (int32_t)i;
-    }
-  }
+  tamx::scan_phase2<1024, int32_t>(tamx::kScanBlocks, part, partial);
+}
+
+__global__ __launch_bounds__(kPlanThreads) void plan_items_fill_kernel(
+    const tamx_gemm_task* __restrict__ tasks, const int64_t* __restrict__ counters,
+    int bm, int bn, int64_t capacity, const int32_t* __restrict__ partial,
+    int32_t* __restrict__ prefix, int32_t* __restrict__ item_task,
+    int64_t item_capacity) {
+  __shared__ int32_t part[kPlanThreads];
+  const int64_t n = planned_tasks(counters, capacity);
+  const int32_t total = partial[tamx::kScanBlocks];
+  const bool map = total <= item_capacity;
+  auto count = [&](int64_t i) { return task_items(tasks, i, bm, bn); };
+  tamx::scan_phase3<kPlanThreads, int32_t>(
+      n, part, count,
+      [&](int64_t i, int32_t q0) {
+        prefix[i] = q0;
+        if (map) {
+          const int32_t q1 = q0 + count(i);
+          for (int32_t q = q0; q < q1; ++q) item_task[q] = (int32_t)i;
+        }
+      },
+      partial);
+  if (blockIdx.x == 0 && threadIdx.x == 0) prefix[n] = total;
 }
 
 __device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -394,10 +420,7 @@ int launch_cfg(tamx_context* ctx, tamx_worklist* wl, const GemmParams<T>& prm,
     }
   }
   const int grid = ctx->num_cus * (wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ);
-  plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, BM, BN,
-                                       wl->item_prefix, wl->item_task,
-                                       wl->task_capacity, wl->item_capacity);
-  TAMX_LAUNCH_CHECK();
+  if (int rc = tamx::plan_items(wl, BM, BN, s)) return rc;
   kern<<<grid, kThreads, 0, s>>>(prm);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
@@ -443,11 +466,25 @@ int grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
   TAMX_CFG(128, 64, 32)
   TAMX_CFG(64, 128, 32)
 #undef TAMX_CFG
-  tamx::set_error("unsupported macro tile: see tamx_gemm_config_list");
+  tamx::set_error("unsupported macro tile: see tamx_gemm_config_get");
   return TAMX_ERR_UNSUPPORTED;
 }
 
 }  // namespace
+
+int tamx::plan_items(tamx_worklist* wl, int bm, int bn, hipStream_t s) {
+  auto* partial = static_cast<int32_t*>(wl->scan_partials);
+  plan_items_sum_kernel<<<tamx::kScanBlocks, kPlanThreads, 0, s>>>(
+      wl->tasks, wl->counters, bm, bn, wl->task_capacity, partial);
+  TAMX_LAUNCH_CHECK();
+  plan_items_offsets_kernel<<<1, 1024, 0, s>>>(partial);
+  TAMX_LAUNCH_CHECK();
+  plan_items_fill_kernel<<<tamx::kScanBlocks, kPlanThreads, 0, s>>>(
+      wl->tasks, wl->counters, bm, bn, wl->task_capacity, partial, wl->item_prefix,
+      wl->item_task, wl->item_capacity);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
 
 extern "C" {
 

@@ -41,16 +41,26 @@ struct tamx_worklist {
   int32_t* tile_npairs = nullptr;
   int64_t* tile_volume = nullptr;
   int64_t* tile_flops = nullptr;
-  int32_t* tile_task = nullptr;
-  int32_t* tile_pair_begin = nullptr;
-  int64_t* tile_c_begin = nullptr;
   // scratch of the grouped GEMM: macro-tile prefix over tasks and the
   // persistent-kernel queue (next item, finished workgroups)
   int32_t* item_prefix = nullptr;  // task_capacity + 1
   int32_t* item_task = nullptr;    // item -> task map, item_capacity entries
   int64_t item_capacity = 0;
   int32_t* queue = nullptr;        // 16 int32 (per-XCD next item, done count), zero between launches
+  // partial sums of the multi-workgroup scans: max(kScanBlocks, grid tiles
+  // / 256 + 1) + 1 entries of 32 bytes
+  void* scan_partials = nullptr;
 };
+
+namespace tamx {
+constexpr int kScanBlocks = 256;  // workgroups of the multi-workgroup scans
+
+// Fills wl->item_prefix (macro tiles of bm x bn per task, exclusive prefix,
+// total at [ntasks]) and wl->item_task (item -> task) when the total fits
+// wl->item_capacity.  Three launches on `s`; reads the task count from the
+// work list's counters, so the sequence is graph-capturable.
+int plan_items(tamx_worklist* wl, int bm, int bn, hipStream_t s);
+}
 
 #define TAMX_CHECK_HIP(expr)                                                   \
   do {                                                                         \

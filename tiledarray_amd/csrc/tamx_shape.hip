@@ -227,6 +227,23 @@ __global__ void shape_gemm_kernel(ShapeGemmParams p) {
 
 // ---------------------------------------------------- work-list builder --
 
+// Sums scanned to place every non-zero result tile: (task flag, pair count,
+// element count, flops).
+struct Quad {
+  int64_t t, p, c, f;
+  __device__ Quad(int v = 0) : t(v), p(v), c(v), f(v) {}
+  __device__ Quad(int64_t t_, int64_t p_, int64_t c_, int64_t f_) : t(t_), p(p_), c(c_), f(f_) {}
+  __device__ Quad& operator+=(const Quad& o) {
+    t += o.t;
+    p += o.p;
+    c += o.c;
+    f += o.f;
+    return *this;
+  }
+};
+
+constexpr int kWlThreads = 256;  // result tiles per workgroup of count / fill
+
 struct WlParams {
   tamx_contraction_desc d;
   tamx_gemm_task* tasks;
@@ -235,9 +252,7 @@ struct WlParams {
   int32_t* tile_npairs;
   int64_t* tile_volume;
   int64_t* tile_flops;
-  int32_t* tile_task;
-  int32_t* tile_pair_begin;
-  int64_t* tile_c_begin;
+  Quad* partial;  // one sum per count workgroup, then the exclusive offsets
   int64_t task_capacity, pair_capacity;
 };
 
@@ -257,70 +272,63 @@ __device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
   return true;
 }
 
-// Pass 1: per result tile, the number of contributing pairs, the element
-// count and the flops (zero for result tiles screened out by the shape).
-__global__ void wl_count_kernel(WlParams p) {
-#pragma clang fp contract(off)
-  const tamx_contraction_desc& d = p.d;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = d.Mt * d.Nt;
-  if (g >= per * d.nbatch) return;
-  const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
-  int32_t np = 0;
-  int64_t vol = 0, flops = 0;
-  if (!(d.c_norms[g] < d.c_threshold)) {
-    const int64_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
-    vol = m * n;
-    const float thr_k = d.pair_threshold > 0.0f
-                            ? d.pair_threshold / (float)d.Kt
-                            : 0.0f;
-    for (int64_t k = 0; k < d.Kt; ++k) {
-      const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
-      const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
-      if (pair_kept(d, ai, bi, thr_k)) {
-        ++np;
-        flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k];
-      }
-    }
-  }
-  p.tile_npairs[g] = np;
-  p.tile_volume[g] = vol;
-  p.tile_flops[g] = flops;
+__device__ inline Quad tile_quad(const WlParams& p, int64_t g) {
+  const int64_t v = p.tile_volume[g];
+  return Quad(v > 0 ? 1 : 0, p.tile_npairs[g], v, p.tile_flops[g]);
 }
 
-// Pass 2 (one workgroup): one exclusive scan of (task flag, pair count,
-// element count, flops) gives each non-zero result tile its task index,
-// first pair and place in C (row-major tile order).
-struct Quad {
-  int64_t t, p, c, f;
-  __device__ Quad(int v = 0) : t(v), p(v), c(v), f(v) {}
-  __device__ Quad(int64_t t_, int64_t p_, int64_t c_, int64_t f_) : t(t_), p(p_), c(c_), f(f_) {}
-  __device__ Quad& operator+=(const Quad& o) {
-    t += o.t;
-    p += o.p;
-    c += o.c;
-    f += o.f;
-    return *this;
+// Pass 1: per result tile, the number of contributing pairs, the element
+// count and the flops (zero for result tiles screened out by the shape);
+// per workgroup, the sum of its tiles' Quads.
+__global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
+#pragma clang fp contract(off)
+  const tamx_contraction_desc& d = p.d;
+  const int64_t g = (int64_t)blockIdx.x * kWlThreads + threadIdx.x;
+  const int64_t per = d.Mt * d.Nt;
+  int32_t np = 0;
+  int64_t vol = 0, flops = 0;
+  if (g < per * d.nbatch) {
+    const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
+    if (!(d.c_norms[g] < d.c_threshold)) {
+      const int64_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
+      vol = m * n;
+      const float thr_k = d.pair_threshold > 0.0f
+                              ? d.pair_threshold / (float)d.Kt
+                              : 0.0f;
+      for (int64_t k = 0; k < d.Kt; ++k) {
+        const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+        const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
+        if (pair_kept(d, ai, bi, thr_k)) {
+          ++np;
+          flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k];
+        }
+      }
+    }
+    p.tile_npairs[g] = np;
+    p.tile_volume[g] = vol;
+    p.tile_flops[g] = flops;
   }
-};
-
-__global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p) {
   // raw storage: __shared__ objects cannot have constructors
+  __shared__ __align__(16) unsigned char red_raw[sizeof(Quad) * kWlThreads];
+  Quad* red = reinterpret_cast<Quad*>(red_raw);
+  red[threadIdx.x] = Quad(vol > 0 ? 1 : 0, np, vol, flops);
+  __syncthreads();
+  for (int w = kWlThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) p.partial[blockIdx.x] = red[0];
+}
+
+// Pass 2 (one workgroup): exclusive scan of the per-workgroup sums, the
+// totals and the capacity check.
+__global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p, int nparts) {
   __shared__ __align__(16) unsigned char part_raw[sizeof(Quad) * 1024];
   Quad* part = reinterpret_cast<Quad*>(part_raw);
   const tamx_contraction_desc& d = p.d;
-  const int64_t n = d.Mt * d.Nt * d.nbatch;
   const Quad tot = tamx::block_exclusive_scan<1024, Quad>(
-      n, part,
-      [&](int64_t i) -> Quad {
-        const int64_t v = p.tile_volume[i];
-        return Quad(v > 0 ? 1 : 0, p.tile_npairs[i], v, p.tile_flops[i]);
-      },
-      [&](int64_t i, const Quad& v) {
-        p.tile_task[i] = (int32_t)v.t;
-        p.tile_pair_begin[i] = (int32_t)v.p;
-        p.tile_c_begin[i] = v.c;
-      });
+      nparts, part, [&](int64_t i) -> Quad { return p.partial[i]; },
+      [&](int64_t i, const Quad& v) { p.partial[i] = v; });
   if (threadIdx.x == 0) {
     int64_t status = 0;
     if (tot.t > p.task_capacity || tot.p > p.pair_capacity || tot.c > d.c_capacity)
@@ -337,13 +345,24 @@ __global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p) {
   }
 }
 
-// Pass 3: write tasks, pairs (k ascending) and the C offsets.
-__global__ void wl_fill_kernel(WlParams p) {
+// Pass 3: each workgroup rescans its own 256 tiles from its offset and
+// writes tasks (row-major tile order), pairs (k ascending) and C offsets.
+__global__ __launch_bounds__(kWlThreads) void wl_fill_kernel(WlParams p) {
 #pragma clang fp contract(off)
   const tamx_contraction_desc& d = p.d;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = d.Mt * d.Nt;
+  const int64_t first = (int64_t)blockIdx.x * kWlThreads;
+  const int64_t g = first + threadIdx.x;
+  const int64_t nloc = min<int64_t>(kWlThreads, per * d.nbatch - first);
+  __shared__ __align__(16) unsigned char part_raw[sizeof(Quad) * kWlThreads];
+  Quad* part = reinterpret_cast<Quad*>(part_raw);
+  Quad mine(0);
+  // one tile per thread: thread t receives the prefix of tile first + t
+  tamx::block_exclusive_scan<kWlThreads, Quad>(
+      nloc, part, [&](int64_t i) -> Quad { return tile_quad(p, first + i); },
+      [&](int64_t, const Quad& v) { mine = v; });
   if (g >= per * d.nbatch) return;
+  mine += p.partial[blockIdx.x];
   const bool ok = p.counters[TAMX_WL_STATUS] == 0;
   if (!ok || p.tile_volume[g] == 0) {
     d.c_off[g] = -1;
@@ -351,10 +370,10 @@ __global__ void wl_fill_kernel(WlParams p) {
   }
   const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
   const int32_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
-  const int64_t coff = p.tile_c_begin[g];
+  const int64_t coff = mine.c;
   d.c_off[g] = coff;
-  const int32_t t = p.tile_task[g];
-  const int32_t pb = p.tile_pair_begin[g];
+  const int32_t t = (int32_t)mine.t;
+  const int32_t pb = (int32_t)mine.p;
   tamx_gemm_task task;
   task.c_off = coff;
   task.m = m;
@@ -539,18 +558,16 @@ int tamx_contraction_worklist(tamx_context* ctx,
              wl->tile_npairs,
              wl->tile_volume,
              wl->tile_flops,
-             wl->tile_task,
-             wl->tile_pair_begin,
-             wl->tile_c_begin,
+             static_cast<Quad*>(wl->scan_partials),
              wl->task_capacity,
              wl->pair_capacity};
   hipStream_t s = tamx::as_stream(stream);
-  const unsigned g = (unsigned)std::max<int64_t>(1, (grid + 255) / 256);
-  wl_count_kernel<<<g, 256, 0, s>>>(p);
+  const unsigned g = (unsigned)std::max<int64_t>(1, (grid + kWlThreads - 1) / kWlThreads);
+  wl_count_kernel<<<g, kWlThreads, 0, s>>>(p);
   TAMX_LAUNCH_CHECK();
-  wl_scan_kernel<<<1, 1024, 0, s>>>(p);
+  wl_scan_kernel<<<1, 1024, 0, s>>>(p, (int)g);
   TAMX_LAUNCH_CHECK();
-  wl_fill_kernel<<<g, 256, 0, s>>>(p);
+  wl_fill_kernel<<<g, kWlThreads, 0, s>>>(p);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
 }

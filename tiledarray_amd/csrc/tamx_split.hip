@@ -1,27 +1,35 @@
 // Split-precision contraction: FP64 tiles are split into BF16 limbs with one
 // power-of-two scale per tile, the limb products run on BF16 MFMA with FP32
-// accumulation, and every FP32 partial sum is folded into FP64 before it can
-// round.
+// accumulation, and the FP32 partial sums are folded into FP64 before the
+// ones that matter can round.
 //
 // Splitting (one pass per operand, tamx_split_f64):
 //   sigma = exponent of the tile's max |a| plus one, so x = a * 2^-sigma is in
-//   (-1, 1);  l_0 = rint(256 x), x_1 = 256 x - l_0, l_1 = rint(256 x_1), ...
-//   Every step is exact in FP64; |l_0| <= 256, |l_s| <= 128 for s >= 1, so
-//   every limb is an integer that BF16 holds exactly, and
-//   a = 2^sigma * sum_s l_s 2^-8(s+1) + r with |r| <= 2^(sigma - 8S - 1).
+//   (-1, 1).  The first limb takes 8 bits, every later limb 9:
+//     l_0 = rint(256 x),          x_1 = 256 x - l_0          (|x_1| <= 1/2)
+//     l_s = rint(512 x_s),        x_{s+1} = 512 x_s - l_s    (s >= 1)
+//   Every step is exact in FP64 and |l_s| <= 256, so each l_s is an integer
+//   that BF16 holds exactly (8 significant bits).  Limb s is stored as
+//   L_s = l_s 2^-9s (exact: only the exponent moves), so that
+//     a = 2^(sigma-8) * sum_s L_s + r,   |r| <= 2^(sigma - 9S)
+//   and a limb product L_s L_t carries its own level weight 2^-9(s+t).
 //   Limbs are written in the layout the MFMA wants: the logical op(A) tile as
 //   rows x K and op(B) as columns x K, K contiguous, rows padded to 16 and K
 //   to 32 with zeros, so the GEMM needs no bounds checks and 16-byte loads.
 //
 // Multiplying (tamx_grouped_gemm_split): for a tile pair the limb products
-// with s + t < S (the triangular set, S(S+1)/2 MFMA products) are
-// accumulated per level g = s + t in FP32.  Products are integers below
-// 2^16 and a level holds at most 2^16 + 2^15 + 2^14 + ... per k, so 128
-// consecutive k stay below 2^24 and every FP32 sum is exact.  Every 128 k and
-// at each pair boundary the levels are folded in FP64:
-//   C += 2^(sigma_A + sigma_B) * sum_g 2^-8(g+2) * P_g.
-// The result error is the truncation of the limb expansion (about
-// 2^-8S relative to the tiles' maxima) plus FP64 rounding.
+// with s + t < S (the triangular set, S(S+1)/2 MFMA products) go to three
+// FP32 accumulators:
+//   P_0 = sum L_0 L_0       integers, |.| <= 2^16 per k
+//   P_1 = sum L_0 L_1 + L_1 L_0   multiples of 2^-9, |.| <= 2^8 per k
+//   P_T = sum of the levels s + t >= 2
+// Over 128 consecutive k, P_0 and P_1 stay within 24 significant bits, so
+// both are exact; P_T rounds, but at 2^-24 of a level that is itself 2^-18
+// below P_0.  Every 128 k and at each pair boundary the three are folded in
+// FP64:  C += 2^(sigma_A + sigma_B - 16) * (P_0 + P_1 + P_T).
+// The result error is the truncation of the limb expansion (about 2^-9S
+// relative to the tiles' maxima, from the dropped remainders and the dropped
+// levels s + t >= S) plus FP64 rounding.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -154,14 +162,18 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
     for (int y = ty; y < kSplitTS; y += 8) {
       const int r = r0 + y;
       if (r >= t.rows_pad) continue;
-      double x = ldexp(tileb[y][tx], -t.sigma);
+      const double x = ldexp(tileb[y][tx], -t.sigma);
       const int64_t o = t.limb_off + (int64_t)r * t.k_pad + k0 + tx;
+      double xs = x * 256.0;
+      double l = rint(xs);
+      limbs[o] = (__bf16)(float)l;
+      double rem = xs - l;
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const double xs = x * 256.0;
-        const double l = rint(xs);
-        limbs[o + s * plane] = (__bf16)(float)l;
-        x = xs - l;
+      for (int s = 1; s < S; ++s) {
+        xs = rem * 512.0;
+        l = rint(xs);
+        limbs[o + s * plane] = (__bf16)(float)ldexp(l, -9 * s);
+        rem = xs - l;
       }
     }
     __syncthreads();
@@ -209,30 +221,6 @@ __device__ inline int64_t uni64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-__global__ __launch_bounds__(1024) void split_plan_items_kernel(
-    const tamx_gemm_task* __restrict__ tasks, const int64_t* __restrict__ counters,
-    int32_t* __restrict__ prefix, int32_t* __restrict__ item_task, int64_t capacity,
-    int64_t item_capacity) {
-  __shared__ int32_t part[1024];
-  int64_t n = counters[TAMX_WL_NTASKS];
-  if (n < 0 || n > capacity || counters[TAMX_WL_STATUS] != 0) n = 0;
-  auto count = [&](int64_t i) -> int32_t {
-    const tamx_gemm_task t = tasks[i];
-    return ((t.m + SBM - 1) / SBM) * ((t.n + SBN - 1) / SBN);
-  };
-  const int32_t total = tamx::block_exclusive_scan<1024, int32_t>(
-      n, part, count, [&](int64_t i, int32_t v) { prefix[i] = v; });
-  if (threadIdx.x == 0) prefix[n] = total;
-  if (total <= item_capacity) {
-    const int64_t seg = (n + 1023) / 1024;
-    const int64_t b = min<int64_t>(n, threadIdx.x * seg), e = min<int64_t>(n, b + seg);
-    for (int64_t i = b; i < e; ++i) {
-      const int32_t q0 = prefix[i], q1 = q0 + count(i);
-      for (int32_t q = q0; q < q1; ++q) item_task[q] = (int32_t)i;
-    }
-  }
-}
-
 // LDS layout of one operand chunk: S planes x 64 rows x 32 k of BF16, as
 // 16-byte units (8 k each); unit q of row r sits at r * 4 + (q ^ (r & 3)) so
 // the 16 rows a quarter-wave reads fall in different banks.
@@ -240,10 +228,9 @@ __device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ (r & 3)); }
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
-  constexpr int NG = S;  // levels g = s + t < S
-  // LDS holds only the limb chunks (S * 16 KB, so that two workgroups fit a
-  // CU at S = 5); the grabbed item is passed through the first word of sA
-  // while the buffers are idle between items
+  constexpr int NG = S >= 3 ? 3 : S;  // accumulators: P_0, P_1, P_T
+  // LDS holds only the limb chunks (S * 16 KB); the grabbed item is passed
+  // through the first word of sA while the buffers are idle between items
   __shared__ bf8_t sA[2][S][SBM * 4];
   __shared__ bf8_t sB[2][S][SBN * 4];
   int* s_item = reinterpret_cast<int*>(&sA[0][0][0]);
@@ -321,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
 
     const int pend = t_pb + t_pc;
 
-    // load cursor: pair lp, chunk lk0
+    // load cursor: pair lp, chunk lk0; runs two chunks ahead of compute
     int lp = t_pb, lk0 = 0, lkp = 0;
     int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0;
     auto set_pair = [&](int pi) {
@@ -337,7 +324,13 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> 3;
     };
     if (lp < pend) set_pair(lp);
-    auto advance = [&]() {
+    auto load_chunk = [&](bf8_t (&ra)[S], bf8_t (&rb)[S]) {
+      const int64_t q = (lk0 >> 3) + st_q;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        ra[s] = p.LA[la_row + s * la_plane + q];
+        rb[s] = p.LB[lb_row + s * lb_plane + q];
+      }
       lk0 += SBK;
       if (lk0 >= lkp) {
         ++lp;
@@ -345,16 +338,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         if (lp < pend) set_pair(lp);
       }
     };
-    bf8_t ra[S], rb[S];
-    auto load_chunk = [&]() {
-      const int64_t q = (lk0 >> 3) + st_q;
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        ra[s] = p.LA[la_row + s * la_plane + q];
-        rb[s] = p.LB[lb_row + s * lb_plane + q];
-      }
-    };
-    auto store_chunk = [&](int buf) {
+    auto store_chunk = [&](int buf, const bf8_t (&ra)[S], const bf8_t (&rb)[S]) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         sA[buf][s][lds_unit(st_r, st_q)] = ra[s];
@@ -363,27 +347,29 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     };
     auto compute_chunk = [&](int buf) {
       if (!wave_active) return;
-      bf8_t af[2][S], bf[2][S];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int s = 0; s < S; ++s) af[i][s] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
+      bf8_t bf[2][S];
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int s = 0; s < S; ++s) bf[j][s] = sB[buf][s][lds_unit(wc + j * 16 + lrow, lq)];
+        for (int t = 0; t < S; ++t) bf[j][t] = sB[buf][t][lds_unit(wc + j * 16 + lrow, lq)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int s = 0; s < S; ++s) {
+        bf8_t af[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
 #pragma unroll
-          for (int s = 0; s < S; ++s)
+        for (int t = 0; t < S - s; ++t) {
+          const int g = s + t < 2 ? s + t : 2;
 #pragma unroll
-            for (int t = 0; t < S - s; ++t)
-              lvl[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  af[i][s], bf[j][t], lvl[s + t][i][j], 0, 0, 0);
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+              lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
+                                                                     lvl[g][i][j], 0, 0, 0);
+        }
+      }
     };
-    // fold the FP32 levels of the pair being computed (exponent e) into FP64
+    // fold the FP32 accumulators of the pair being computed (exponent e)
     auto fold = [&](int e) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -394,10 +380,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
             double tsum = 0.0;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) {
-              tsum += ldexp((double)lvl[g][i][j][v], -8 * (g + 2));
+              tsum += (double)lvl[g][i][j][v];
               lvl[g][i][j][v] = 0.0f;
             }
-            acc[i][j][v] += ldexp(tsum, e);
+            acc[i][j][v] += ldexp(tsum, e - 16);
           }
     };
 
@@ -427,24 +413,52 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       }
     };
 
-    if (lp < pend) {
-      load_chunk();
-      advance();
-      store_chunk(0);
-      __syncthreads();
-      int buf = 0;
-      for (;;) {
-        const bool has_next = lp < pend;
-        if (has_next) {
-          load_chunk();
-          advance();
-        }
-        compute_chunk(buf);
-        after_compute();
-        if (!has_next) break;
-        store_chunk(buf ^ 1);
+    // Pipeline: registers R0/R1 hold chunks c+1 and c+2 in flight while
+    // chunk c is multiplied out of LDS buffer c % 2.  One barrier per chunk.
+    // Above four limbs the second register set would cost occupancy, so the
+    // loads run one chunk ahead instead.
+    bf8_t ra0[S], rb0[S], ra1[S], rb1[S];
+    if constexpr (S > 4) {
+      if (lp < pend) {
+        load_chunk(ra0, rb0);
+        store_chunk(0, ra0, rb0);
         __syncthreads();
-        buf ^= 1;
+        int buf = 0;
+        for (;;) {
+          const bool has_next = lp < pend;
+          if (has_next) load_chunk(ra0, rb0);
+          compute_chunk(buf);
+          after_compute();
+          if (!has_next) break;
+          store_chunk(buf ^ 1, ra0, rb0);
+          __syncthreads();
+          buf ^= 1;
+        }
+      }
+    } else if (lp < pend) {
+      load_chunk(ra0, rb0);                // chunk 0
+      const bool have1 = lp < pend;
+      if (have1) load_chunk(ra1, rb1);     // chunk 1
+      store_chunk(0, ra0, rb0);
+      __syncthreads();
+      bool more = have1;  // chunk c + 1 exists
+      for (;;) {
+        bool next2 = lp < pend;
+        if (next2) load_chunk(ra0, rb0);   // chunk c + 2 -> R0
+        compute_chunk(0);
+        after_compute();
+        if (!more) break;
+        store_chunk(1, ra1, rb1);
+        __syncthreads();
+        more = next2;
+        next2 = lp < pend;
+        if (next2) load_chunk(ra1, rb1);   // chunk c + 2 -> R1
+        compute_chunk(1);
+        after_compute();
+        if (!more) break;
+        store_chunk(0, ra0, rb0);
+        __syncthreads();
+        more = next2;
       }
     }
 
@@ -486,10 +500,7 @@ int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& p
   int occ = 0;
   TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
   if (occ < 1) occ = 1;
-  split_plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, wl->item_prefix,
-                                             wl->item_task, wl->task_capacity,
-                                             wl->item_capacity);
-  TAMX_LAUNCH_CHECK();
+  if (int rc = tamx::plan_items(wl, SBM, SBN, s)) return rc;
   kern<<<ctx->num_cus * occ, kThreads, 0, s>>>(prm);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
@@ -753,10 +764,7 @@ int launch_f32x2(tamx_context* ctx, tamx_worklist* wl, const double* A, const do
   int occ = 0;
   TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
   if (occ < 1) occ = 1;
-  split_plan_items_kernel<<<1, 1024, 0, s>>>(wl->tasks, wl->counters, wl->item_prefix,
-                                             wl->item_task, wl->task_capacity,
-                                             wl->item_capacity);
-  TAMX_LAUNCH_CHECK();
+  if (int rc = tamx::plan_items(wl, SBM, SBN, s)) return rc;
   kern<<<ctx->num_cus * occ, kThreads, 0, s>>>(A, B, prm);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;

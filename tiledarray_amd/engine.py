@@ -63,6 +63,57 @@ class Context:
             pass
 
 
+class Graph:
+    """A launch sequence captured through tamx_graph_begin/_end.  `fn` is
+    called once on a side stream under capture; replay() relaunches it on
+    the current stream."""
+
+    def __init__(self, ctx, fn):
+        self.ctx = ctx
+        self.ptr = None
+        side = torch.cuda.Stream(ctx.device)
+        side.wait_stream(torch.cuda.current_stream(ctx.device))
+        with torch.cuda.stream(side):
+            s = ctypes.c_void_p(side.cuda_stream)
+            call("tamx_graph_begin", ctx.ptr, s)
+            p = ctypes.c_void_p()
+            try:
+                fn()
+            except BaseException:
+                # end the capture so the stream is usable again, drop the graph
+                lib().tamx_graph_end(ctx.ptr, s, ctypes.byref(p))
+                if p:
+                    lib().tamx_graph_destroy(p)
+                raise
+            call("tamx_graph_end", ctx.ptr, s, ctypes.byref(p))
+            self.ptr = p
+        torch.cuda.current_stream(ctx.device).wait_stream(side)
+
+    def replay(self):
+        call("tamx_graph_launch", self.ptr, self.ctx.stream())
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_graph_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def autotune(ctx, wl, dtype, op_a, op_b, A, B, C, reps=3, force=False, warmup=True):
+    """Fastest compiled tile configuration for the built work list `wl`
+    (tamx_autotune_grouped_gemm); returns ((bm, bn, bk, wgs_per_cu), ms)."""
+    prec = _lib.PREC_F64 if dtype == torch.float64 else _lib.PREC_F32
+    cfg = _lib.GemmConfig()
+    ms = ctypes.c_float()
+    flags = (_lib.TUNE_FORCE if force else 0) | (0 if warmup else _lib.TUNE_NO_WARMUP)
+    call("tamx_autotune_grouped_gemm", ctx.ptr, wl.ptr, prec, int(op_a), int(op_b),
+         A.data_ptr(), B.data_ptr(), C.data_ptr(), int(reps), flags, ctypes.byref(cfg),
+         ctypes.byref(ms), ctx.stream())
+    return (cfg.bm, cfg.bn, cfg.bk, cfg.wgs_per_cu), ms.value
+
+
 class WorkList:
     def __init__(self, ctx, task_capacity, pair_capacity, grid_capacity):
         self.ctx = ctx
@@ -393,6 +444,16 @@ class Contraction:
             t = self.tile + (16, 0)[len(self.tile) - 2:] if len(self.tile) < 4 else self.tile
             self.tile_cfg = _lib.GemmConfig(*t[:4])
 
+    def tune(self, reps=3, force=False, warmup=True):
+        """Autotunes the tile configuration on the current work list (built
+        by worklist_step) and keeps it; FP64 and FP32 only."""
+        if self.precision not in ("f64", "f32"):
+            raise ValueError("autotuning covers the f64 and f32 grouped GEMMs")
+        tile, ms = autotune(self.ctx, self.wl, self.C.dtype, self.op_a, self.op_b, self.A.pool,
+                            self.B.pool, self.C, reps=reps, force=force, warmup=warmup)
+        self.set_tile(tile)
+        return tile, ms
+
     # -- device steps -------------------------------------------------------
     def permute_step(self):
         """Re-permutes `general` operands from their source arrays."""
@@ -597,17 +658,23 @@ class BatchedContraction:
         return {"ntasks": c[_lib.WL_NTASKS], "npairs": c[_lib.WL_NPAIRS],
                 "celems": c[_lib.WL_CELEMS], "flops": c[_lib.WL_FLOPS]}
 
+    def tune(self, reps=3, force=False, warmup=True):
+        """Autotunes the tile configuration on the current work list (built
+        by worklist_step) and keeps it."""
+        tile, ms = autotune(self.ctx, self.wl, self.dtype, 0, 0, self.A, self.B, self.C,
+                            reps=reps, force=force, warmup=warmup)
+        self.tile_cfg = _lib.GemmConfig(*tile)
+        return tile, ms
+
     def capture(self):
-        """Captures run() into a hipGraph (torch.cuda.CUDAGraph); returns it.
-        Every step is device work with no host synchronisation, no allocation
-        and no memset, so the graph replays the whole contraction."""
-        g = torch.cuda.CUDAGraph()
+        """Captures run() into a graph (tamx_graph_begin/_end); returns a
+        Graph whose replay() relaunches it.  Every step is device work with
+        no host synchronisation, no allocation and no memset, so the graph
+        replays the whole contraction."""
         side = torch.cuda.Stream(self.ctx.device)
         side.wait_stream(torch.cuda.current_stream(self.ctx.device))
         with torch.cuda.stream(side):
             self.run()  # warm caches (occupancy queries) outside the capture
         torch.cuda.current_stream(self.ctx.device).wait_stream(side)
         torch.cuda.synchronize()
-        with torch.cuda.graph(g):
-            self.run()
-        return g
+        return Graph(self.ctx, self.run)
