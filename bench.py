@@ -96,6 +96,39 @@ def time_steps(c, step_fn, steps, warmup):
     return (t1 - t0) * 1e3 / steps, gemm_ms
 
 
+def time_kernel(fn, reps=5):
+    """Mean duration (ms) of fn's launches on the current stream, by events."""
+    import torch
+    fn()
+    ev = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def bandwidth(name_bytes_ms):
+    """Roofline entries of the bandwidth-bound kernels: {name: (bytes, ms)}."""
+    out = {}
+    for name, (nbytes, ms) in name_bytes_ms.items():
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "bytes": int(nbytes), "avg_launch_ms": round(ms, 4),
+                     "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return out
+
+
+def stored_bytes(X):
+    """Bytes of the stored (non-zero) tiles of a tiled array."""
+    import numpy as np
+    vols = X.trange.tile_volumes()
+    return int(np.sum(np.where(X.offsets >= 0, vols, 0))) * X.pool.element_size()
+
+
 def calibrate_copy():
     import torch
     n = 1 << 27  # 2^27 doubles = 1 GiB
@@ -248,6 +281,11 @@ def main():
             }
             del cs
         out["split_precision"] = split
+    if sparse:
+        # screening: every stored element read once per norm pass
+        nb = stored_bytes(A) + stored_bytes(B)
+        ms_n = time_kernel(lambda: [E.tile_norms(ctx, X, out=X.shape_norms) for X in (A, B)])
+        out["bandwidth"] = bandwidth({"tile_norms": (nb, ms_n)})
     print(json.dumps(out), flush=True)
 
 
@@ -320,6 +358,17 @@ def bench_ladder(args):
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,
                      "avg_launch_ms": round(gemm_ms, 4)})
+    # bandwidth-bound kernels: the V permute (each stored element read and
+    # written once), the norm passes over T and V, the limb split
+    bw = {"permute": (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3)),
+          "tile_norms": (stored_bytes(T) + stored_bytes(V),
+                         time_kernel(lambda: [E.tile_norms(ctx, X, out=X.shape_norms)
+                                              for X in (T, V)], reps=3))}
+    if c.nslices:
+        # FP64 read, S BF16 limbs written, for both operands
+        nel = (stored_bytes(T) + stored_bytes(c.B)) // 8
+        bw["split_limbs"] = (nel * (8 + 2 * c.nslices), time_kernel(c.split_step, reps=3))
+    out["bandwidth"] = bandwidth(bw)
     if not args.no_cpu:
         out["cpu_baseline"], out["rel_error"] = ladder_cpu_sample(args, c, T, V)
     print(json.dumps(out), flush=True)
@@ -450,6 +499,12 @@ def bench_tot(args):
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,
                      "avg_launch_ms": round(gemm_ms, 4)})
+    # screening: cell norms (every stored element read once), scaling and
+    # the batched estimate
+    el = w["A"].element_size()
+    nb = (int(np.sum(np.where(w["a_off"] >= 0, c.a_plan.volumes, 0))) +
+          int(np.sum(np.where(w["b_off"] >= 0, c.b_plan.volumes, 0)))) * el
+    out["bandwidth"] = bandwidth({"screening": (nb, time_kernel(c.shape_step))})
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         oracle.set_num_threads(threads)

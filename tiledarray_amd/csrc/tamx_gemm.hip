@@ -212,8 +212,14 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     const int r0 = (local / tiles_n) * BM, c0 = (local % tiles_n) * BN;
     const int mrem = min(BM, t_m - r0), nrem = min(BN, t_n - c0);
     // a wave whose quadrant lies wholly outside a ragged edge skips its
-    // MFMAs (the condition is uniform across the wave)
+    // MFMAs, and inside a quadrant every 16 x 16 block that does (the
+    // conditions are uniform across the wave)
     const bool wave_active = (wr < mrem) && (wc < nrem);
+    bool row_on[FM], col_on[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) row_on[i] = wr + i * 16 < mrem;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) col_on[j] = wc + j * 16 < nrem;
 
     acc_t acc[FM][FN];
 #pragma unroll
@@ -331,7 +337,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j)
+            if (row_on[i] && col_on[j]) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
       }
     };
 

@@ -32,6 +32,7 @@
 // levels s + t >= S) plus FP64 rounding.
 #include <algorithm>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "tamx_internal.hpp"
@@ -345,29 +346,41 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         sB[buf][s][lds_unit(st_r, st_q)] = rb[s];
       }
     };
-    auto compute_chunk = [&](int buf) {
-      if (!wave_active) return;
-      bf8_t bf[2][S];
+    // NI x NJ of the wave's 2 x 2 blocks hold rows and columns of the tile
+    // (a ragged edge leaves the rest empty; the split is uniform per wave)
+    const int ni = wr + 16 < mrem ? 2 : 1, nj = wc + 16 < nrem ? 2 : 1;
+    auto compute_blocks = [&](int buf, auto NI_, auto NJ_) {
+      constexpr int NI = decltype(NI_)::value, NJ = decltype(NJ_)::value;
+      bf8_t bf[NJ][S];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int t = 0; t < S; ++t) bf[j][t] = sB[buf][t][lds_unit(wc + j * 16 + lrow, lq)];
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        bf8_t af[2];
+        bf8_t af[NI];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
+        for (int i = 0; i < NI; ++i) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
 #pragma unroll
         for (int t = 0; t < S - s; ++t) {
           const int g = s + t < 2 ? s + t : 2;
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
+          for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < NJ; ++j)
               lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
                                                                      lvl[g][i][j], 0, 0, 0);
         }
       }
+    };
+    using one = std::integral_constant<int, 1>;
+    using two = std::integral_constant<int, 2>;
+    auto compute_chunk = [&](int buf) {
+      if (!wave_active) return;
+      if (ni == 2 && nj == 2) compute_blocks(buf, two{}, two{});
+      else if (ni == 2) compute_blocks(buf, two{}, one{});
+      else if (nj == 2) compute_blocks(buf, one{}, two{});
+      else compute_blocks(buf, one{}, one{});
     };
     // fold the FP32 accumulators of the pair being computed (exponent e)
     auto fold = [&](int e) {

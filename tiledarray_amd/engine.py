@@ -153,6 +153,7 @@ class NormPlan:
              offsets.size, ctypes.byref(p))
         self.ptr = p
         self.ntiles = offsets.size
+        self.volumes = volumes
 
     def __del__(self):
         try:
