@@ -24,8 +24,10 @@ pytestmark = pytest.mark.gpu
 # relative Frobenius error bounds per limb count (the first limb carries 8
 # bits, every later one 9; the bounds leave ~30x headroom over a numpy
 # simulation of the same expansion: U[-1,1] tiles 1.5e-8 / 3.3e-11 / 4.9e-13
-# at 3 / 4 / 5 limbs, ladder tiles 3.6e-7 / 1.2e-9 / 6.0e-12)
-SPLIT_TOL = {2: 1e-3, 3: 1e-6, 4: 1e-9, 5: 1e-11, 6: 1e-12}
+# at 3 / 4 / 5 limbs, ladder tiles 3.6e-7 / 1.2e-9 / 6.0e-12).  From five
+# limbs on, the FP32 rounding of the lower-level accumulator (summed over
+# 128 k before each FP64 fold) sets a floor near 2e-12.
+SPLIT_TOL = {2: 1e-3, 3: 1e-6, 4: 1e-9, 5: 1e-11, 6: 1e-11}
 LADDER_TOL = {3: 1e-5, 4: 3e-8, 5: 2e-10}
 
 
