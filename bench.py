@@ -181,17 +181,21 @@ def main():
         for X in (A, B):
             sizes[id(X)] = E._sizes_vector(X.trange, X.pool.device)
 
-    def screening():
+    # per-tile max |a| from the norm pass, for the limb split's scales
+    maxabs = {id(X): torch.zeros(X.trange.ntiles, dtype=torch.float64, device=X.pool.device)
+              for X in (A, B)}
+
+    def screening(with_max):
         for X in (A, B):
-            E.tile_norms(ctx, X, out=X.shape_norms)
+            E.tile_norms(ctx, X, out=X.shape_norms, maxabs=maxabs[id(X)] if with_max else None)
             E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
 
     def make_step(c):
         def step(events):
             if sparse:
-                screening()
+                screening(bool(c.nslices))
             if c.nslices:
-                c.split_step()
+                c.split_step(maxabs=(maxabs[id(A)], maxabs[id(B)]) if sparse else None)
             c.shape_step()
             c.worklist_step()
             if events is not None:
@@ -319,15 +323,21 @@ def bench_ladder(args):
     # the permuted V's shape follows the source's; refresh it each step
     perm_norms = c.B.shape_norms
 
+    inv = [int(i) for i in np.argsort(list(c.plan.right_perm[:4]))]
+    maxabs = {id(X): torch.zeros(X.trange.ntiles, dtype=torch.float64, device=X.pool.device)
+              for X in (T, V)}
+    perm_max = torch.zeros(V.trange.ntiles, dtype=torch.float64, device=V.pool.device)
+
     def step(events):
         for X in (T, V):
-            E.tile_norms(ctx, X, out=X.shape_norms)
+            E.tile_norms(ctx, X, out=X.shape_norms,
+                         maxabs=maxabs[id(X)] if c.nslices else None)
             E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
         c.permute_step()
-        inv = [int(i) for i in np.argsort(list(c.plan.right_perm[:4]))]
         perm_norms.copy_(V.shape_norms.view(V.trange.tiles_shape).permute(*inv).reshape(-1))
         if c.nslices:
-            c.split_step()
+            perm_max.copy_(maxabs[id(V)].view(V.trange.tiles_shape).permute(*inv).reshape(-1))
+            c.split_step(maxabs=(maxabs[id(T)], perm_max))
         c.shape_step()
         c.worklist_step()
         if events is not None:
@@ -367,7 +377,9 @@ def bench_ladder(args):
     if c.nslices:
         # FP64 read, S BF16 limbs written, for both operands
         nel = (stored_bytes(T) + stored_bytes(c.B)) // 8
-        bw["split_limbs"] = (nel * (8 + 2 * c.nslices), time_kernel(c.split_step, reps=3))
+        bw["split_limbs"] = (nel * (8 + 2 * c.nslices),
+                             time_kernel(lambda: c.split_step(maxabs=(maxabs[id(T)], perm_max)),
+                                         reps=3))
     out["bandwidth"] = bandwidth(bw)
     if not args.no_cpu:
         out["cpu_baseline"], out["rel_error"] = ladder_cpu_sample(args, c, T, V)

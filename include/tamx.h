@@ -182,6 +182,13 @@ int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
 int tamx_norm_plan_destroy(tamx_norm_plan* plan);
 int tamx_tile_norms_f64(tamx_context* ctx, const tamx_norm_plan* plan,
                         const double* pool, float* norms, void* stream);
+/* The same norms (bit-identical), and in the same pass each tile's max |a|
+ * (0 for absent tiles) into maxabs[ntiles] (device double): the scale input
+ * of tamx_split_f64_premax, so the limb split needs no separate max pass.
+ * No reference counterpart (TA's split-free path needs no max). */
+int tamx_tile_norms_max_f64(tamx_context* ctx, const tamx_norm_plan* plan,
+                            const double* pool, float* norms, double* maxabs,
+                            void* stream);
 int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
                         const float* pool, float* norms, void* stream);
 
@@ -339,6 +346,12 @@ int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma);
  * tamx_split_plan_limb_elems elements). */
 int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan,
                    const double* pool, void* limbs, void* stream);
+/* As tamx_split_f64 with each tile's max |a| given (device double, by the
+ * plan's tile ordinals, e.g. from tamx_tile_norms_max_f64): one pass over
+ * the operand instead of two. */
+int tamx_split_f64_premax(tamx_context* ctx, const tamx_split_plan* plan,
+                          const double* pool, const double* tile_maxabs,
+                          void* limbs, void* stream);
 /* Grouped GEMM on split operands: per pair the limb products with
  * s + t < nslices on BF16 MFMA with FP32 accumulation (the s + t = 0 and
  * s + t = 1 sums exact, the lower levels together), folded into FP64 every

@@ -49,6 +49,40 @@ def test_split_integer_exact(ctx, nslices, op_a, op_b):
     np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
 
 
+def test_norms_with_max_and_premax_split(ctx):
+    # the fused norm + max pass gives the same norms bit for bit and each
+    # tile's exact max |a|; the split fed with those maxima writes the same
+    # limbs as the two-pass split, and the contraction is unchanged
+    A, B, _ = W.blocksparse_gemm(n=1200, wlo=24, whi=96, seed=8)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    for X in (A, B):
+        plain = E.tile_norms(ctx, X)
+        mx = torch.full((X.trange.ntiles,), -1.0, dtype=torch.float64, device="cuda")
+        fused = E.tile_norms(ctx, X, maxabs=mx)
+        torch.cuda.synchronize()
+        assert torch.equal(plain, fused)
+        ref = np.zeros(X.trange.ntiles)
+        for t in range(X.trange.ntiles):
+            if X.offsets[t] >= 0:
+                ref[t] = np.abs(X.tile(t).cpu().numpy()).max()
+        np.testing.assert_array_equal(mx.cpu().numpy(), ref)
+        X._mx = mx
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="bf16x4")
+    c.run()
+    torch.cuda.synchronize()
+    limbs = [c.a_split.limbs.clone(), c.b_split.limbs.clone()]
+    C0 = c.C.clone()
+    for sp in (c.a_split, c.b_split):
+        sp.limbs.zero_()
+    c.split_step(maxabs=(A._mx, B._mx))
+    c.gemm_step()
+    torch.cuda.synchronize()
+    assert torch.equal(c.a_split.limbs, limbs[0]) and torch.equal(c.b_split.limbs, limbs[1])
+    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
+    assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
+
+
 def test_split_sigma_is_tile_exponent(ctx):
     rng = np.random.default_rng(5)
     tr = TiledRange1([0, 20, 45])

@@ -129,6 +129,8 @@ SIGNATURES = {
     "tamx_split_plan_destroy": [_vp],
     "tamx_split_plan_limb_elems": [_vp],
     "tamx_split_plan_sigmas": [_vp, _vp],
+    "tamx_split_f64_premax": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_tile_norms_max_f64": [_vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_split_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_split": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32x2": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp],

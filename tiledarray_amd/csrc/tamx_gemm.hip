@@ -118,6 +118,16 @@ __global__ __launch_bounds__(kPlanThreads) void plan_items_fill_kernel(
   if (blockIdx.x == 0 && threadIdx.x == 0) prefix[n] = total;
 }
 
+// the address out-of-tile lanes load from
+__device__ const double kZeroF64 = 0.0;
+__device__ const float kZeroF32 = 0.0f;
+template <typename T>
+__device__ inline const T* zero_of();
+template <>
+__device__ inline const double* zero_of<double>() { return &kZeroF64; }
+template <>
+__device__ inline const float* zero_of<float>() { return &kZeroF32; }
+
 __device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ inline int64_t uni64(int64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
@@ -248,12 +258,21 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     }
     __syncthreads();
     auto set_pair = [&](int pi) {
-      const tamx_gemm_pair& pp = pairs_in_lds ? s_pairs[pi - t_pb] : p.pairs[pi];
-      a_base = uni64(pp.a_off);
-      b_base = uni64(pp.b_off);
-      pk = uni(pp.k);
-      a_lda = uni(pp.lda);
-      b_ldb = uni(pp.ldb);
+      if (pairs_in_lds) {
+        const tamx_gemm_pair& pp = s_pairs[pi - t_pb];
+        a_base = uni64(pp.a_off);
+        b_base = uni64(pp.b_off);
+        pk = uni(pp.k);
+        a_lda = uni(pp.lda);
+        b_ldb = uni(pp.ldb);
+      } else {
+        const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
+        a_base = uni64(pp.a_off);
+        b_base = uni64(pp.b_off);
+        pk = uni(pp.k);
+        a_lda = uni(pp.lda);
+        b_ldb = uni(pp.ldb);
+      }
 #pragma unroll
       for (int e = 0; e < A_LD; ++e) {
         const int slow = a_slow + e * A_STEP;
@@ -289,8 +308,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
         if (lp < pend) set_pair(lp);
       }
     };
-    // loads of chunk lk0 of the cursor's pair: out-of-tile elements read
-    // the tile's first element and are replaced by zero (no branches)
+    // loads of chunk lk0 of the cursor's pair; out-of-tile elements read a
+    // zero constant, so the loaded registers need no select and nothing
+    // waits for the loads before they are stored to LDS.  Once the cursor
+    // has passed the last pair the loads repeat its first chunk, unused:
+    // every step issues the same loads, which keeps the wait counts exact.
     auto load_chunk = [&](T (&ra)[A_LD], T (&rb)[B_LD]) {
       const int kc = min(BK, pk - lk0);
       const T* __restrict__ a = p.A + a_base;
@@ -301,17 +323,15 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
       for (int e = 0; e < A_LD; ++e) {
         const int kk = TA ? a_slow + e * A_STEP : a_fast;
         const bool ok = a_ok[e] && kk < kc;
-        const T v = a[ok ? a_off[e] + a_k : 0];
-        ra[e] = ok ? v : T(0);
+        ra[e] = *(ok ? tamx::global_ptr(a + a_off[e] + a_k) : tamx::global_ptr(zero_of<T>()));
       }
 #pragma unroll
       for (int e = 0; e < B_LD; ++e) {
         const int kk = TB ? b_fast : b_slow + e * B_STEP;
         const bool ok = b_ok[e] && kk < kc;
-        const T v = b[ok ? b_off[e] + b_k : 0];
-        rb[e] = ok ? v : T(0);
+        rb[e] = *(ok ? tamx::global_ptr(b + b_off[e] + b_k) : tamx::global_ptr(zero_of<T>()));
       }
-      advance();
+      if (lp < pend) advance();
     };
     auto store_chunk = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD]) {
 #pragma unroll
@@ -348,14 +368,14 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     if (lp < pend) {
       load_chunk(ra0, rb0);                    // chunk 0
       const bool have1 = lp < pend;
-      if (have1) load_chunk(ra1, rb1);         // chunk 1
+      load_chunk(ra1, rb1);                    // chunk 1 (if have1)
       store_chunk(0, ra0, rb0);
       __syncthreads();
       bool more = have1;  // chunk c + 1 exists
       for (;;) {
         // even step: compute buffer 0, chunk c+1 is in R1
         bool next2 = lp < pend;
-        if (next2) load_chunk(ra0, rb0);       // chunk c + 2 -> R0
+        load_chunk(ra0, rb0);                  // chunk c + 2 -> R0 (if next2)
         compute_chunk(0);
         if (!more) break;
         store_chunk(1, ra1, rb1);
@@ -363,7 +383,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
         more = next2;
         // odd step: compute buffer 1, chunk c+1 is in R0
         next2 = lp < pend;
-        if (next2) load_chunk(ra1, rb1);       // chunk c + 2 -> R1
+        load_chunk(ra1, rb1);                  // chunk c + 2 -> R1 (if next2)
         compute_chunk(1);
         if (!more) break;
         store_chunk(0, ra0, rb0);

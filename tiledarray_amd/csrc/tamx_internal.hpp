@@ -25,6 +25,23 @@ int activate(const Context* ctx);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Descriptor tables a kernel only reads (work-list tasks and pairs, tile
+// tables, all written by earlier launches), at wave-uniform indices: read
+// through the constant address space, the loads go to the scalar unit and
+// are not ordered behind the vector loads in flight, so fetching the next
+// pair's descriptor never waits for a prefetched operand chunk.
+template <typename T>
+__device__ inline const __attribute__((address_space(4))) T* scalar_ptr(const T* p) {
+  return (const __attribute__((address_space(4))) T*)(p);
+}
+
+// A pointer known to address global memory (so a select between two of
+// them still compiles to global, not flat, loads).
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* global_ptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(p);
+}
+
 }  // namespace tamx
 
 struct tamx_context : tamx::Context {};

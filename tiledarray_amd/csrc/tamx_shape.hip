@@ -27,6 +27,7 @@ struct tamx_norm_plan {
   // tile t owns chunks [tchunk[t], tchunk[t+1]); no chunk for absent tiles
   int64_t* tchunk = nullptr;
   double* partial = nullptr;  // one partial sum per chunk
+  double* partial_max = nullptr;  // one partial max |a| per chunk
 };
 
 namespace {
@@ -36,17 +37,21 @@ constexpr int kNormThreads = 256;
 
 // Partial sums of squares, one workgroup per chunk.  Accumulation in the
 // tile's scalar type: double for FP64 tiles, float for FP32 tiles, as
-// Tensor<T>::squared_norm does (tensor/tensor.h:3768-3776).
-template <typename T>
+// Tensor<T>::squared_norm does (tensor/tensor.h:3768-3776).  With MAX, the
+// same pass also takes each chunk's max |a| (for the limb split's scale).
+template <typename T, bool MAX>
 __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
     const T* __restrict__ pool, const int64_t* __restrict__ cbeg,
-    const int64_t* __restrict__ cend, double* __restrict__ partial) {
+    const int64_t* __restrict__ cend, double* __restrict__ partial,
+    double* __restrict__ partial_max) {
   __shared__ T red[kNormThreads];
+  __shared__ T redm[MAX ? kNormThreads : 1];
   const int64_t c = blockIdx.x;
   const int64_t b = cbeg[c], e = cend[c];
   // eight independent loads in flight per thread, eight partial sums
   constexpr int U = 8;
   T s[U];
+  T m = T(0);
 #pragma unroll
   for (int u = 0; u < U; ++u) s[u] = T(0);
   int64_t i = b + threadIdx.x;
@@ -56,33 +61,52 @@ __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
     for (int u = 0; u < U; ++u) x[u] = pool[i + u * kNormThreads];
 #pragma unroll
     for (int u = 0; u < U; ++u) s[u] += x[u] * x[u];
+    if (MAX) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) m = fmax(m, fabs(x[u]));
+    }
   }
   for (; i < e; i += kNormThreads) {
     const T x = pool[i];
     s[0] += x * x;
+    if (MAX) m = fmax(m, fabs(x));
   }
   T t = T(0);
 #pragma unroll
   for (int u = 0; u < U; ++u) t += s[u];
   red[threadIdx.x] = t;
+  if (MAX) redm[threadIdx.x] = m;
   __syncthreads();
   for (int w = kNormThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      if (MAX) redm[threadIdx.x] = fmax(redm[threadIdx.x], redm[threadIdx.x + w]);
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[c] = (double)red[0];
+  if (threadIdx.x == 0) {
+    partial[c] = (double)red[0];
+    if (MAX) partial_max[c] = (double)redm[0];
+  }
 }
 
 // Combines the chunks of each tile in chunk order: norm = sqrt(sum), cast to
 // the float shape type (make_array stores into Tensor<float>,
-// conversions/make_array.h:170-172).
+// conversions/make_array.h:170-172); with maxabs, also the tile's max |a|.
 template <typename T>
 __global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
                                    const double* __restrict__ partial,
-                                   int64_t ntiles, float* __restrict__ norms) {
+                                   const double* __restrict__ partial_max,
+                                   int64_t ntiles, float* __restrict__ norms,
+                                   double* __restrict__ maxabs) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const int64_t b = tchunk[t], e = tchunk[t + 1];
+  if (maxabs) {
+    double m = 0.0;
+    for (int64_t c = b; c < e; ++c) m = fmax(m, partial_max[c]);
+    maxabs[t] = m;
+  }
   if (b == e) {
     norms[t] = 0.0f;
     return;
@@ -100,20 +124,24 @@ __global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
 
 template <typename T>
 int tile_norms(tamx_context* ctx, const tamx_norm_plan* plan, const T* pool,
-               float* norms, void* stream) {
+               float* norms, double* maxabs, void* stream) {
   TAMX_REQUIRE(plan != nullptr && norms != nullptr, "null argument");
   if (plan->ntiles == 0) return TAMX_OK;
   TAMX_REQUIRE(pool != nullptr || plan->nchunks == 0, "null pool");
   if (int rc = tamx::activate(ctx)) return rc;
   hipStream_t s = tamx::as_stream(stream);
   if (plan->nchunks > 0) {
-    norm_partial_kernel<T><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
-        pool, plan->cbeg, plan->cend, plan->partial);
+    if (maxabs)
+      norm_partial_kernel<T, true><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
+          pool, plan->cbeg, plan->cend, plan->partial, plan->partial_max);
+    else
+      norm_partial_kernel<T, false><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
+          pool, plan->cbeg, plan->cend, plan->partial, nullptr);
     TAMX_LAUNCH_CHECK();
   }
   const unsigned grid = (unsigned)((plan->ntiles + 255) / 256);
-  norm_finish_kernel<T><<<grid, 256, 0, s>>>(plan->tchunk, plan->partial,
-                                              plan->ntiles, norms);
+  norm_finish_kernel<T><<<grid, 256, 0, s>>>(plan->tchunk, plan->partial, plan->partial_max,
+                                              plan->ntiles, norms, maxabs);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
 }
@@ -443,6 +471,7 @@ int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
   alloc(&plan->cend, nc * sizeof(int64_t));
   alloc(&plan->ctile, nc * sizeof(int32_t));
   alloc(&plan->partial, nc * sizeof(double));
+  alloc(&plan->partial_max, nc * sizeof(double));
   alloc(&plan->tchunk, (ntiles + 1) * sizeof(int64_t));
   if (e == hipSuccess && !cbeg.empty()) {
     e = hipMemcpy(plan->cbeg, cbeg.data(), cbeg.size() * sizeof(int64_t), hipMemcpyHostToDevice);
@@ -464,7 +493,8 @@ int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
 
 int tamx_norm_plan_destroy(tamx_norm_plan* plan) {
   if (!plan) return TAMX_OK;
-  void* ptrs[] = {plan->cbeg, plan->cend, plan->ctile, plan->partial, plan->tchunk};
+  void* ptrs[] = {plan->cbeg, plan->cend, plan->ctile, plan->partial, plan->partial_max,
+                  plan->tchunk};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete plan;
@@ -473,12 +503,19 @@ int tamx_norm_plan_destroy(tamx_norm_plan* plan) {
 
 int tamx_tile_norms_f64(tamx_context* ctx, const tamx_norm_plan* plan,
                         const double* pool, float* norms, void* stream) {
-  return tile_norms<double>(ctx, plan, pool, norms, stream);
+  return tile_norms<double>(ctx, plan, pool, norms, nullptr, stream);
+}
+
+int tamx_tile_norms_max_f64(tamx_context* ctx, const tamx_norm_plan* plan,
+                            const double* pool, float* norms, double* maxabs,
+                            void* stream) {
+  TAMX_REQUIRE(maxabs != nullptr, "null maxabs");
+  return tile_norms<double>(ctx, plan, pool, norms, maxabs, stream);
 }
 
 int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
                         const float* pool, float* norms, void* stream) {
-  return tile_norms<float>(ctx, plan, pool, norms, stream);
+  return tile_norms<float>(ctx, plan, pool, norms, nullptr, stream);
 }
 
 int tamx_shape_scale(tamx_context* ctx, float* norms, int rank,

@@ -59,7 +59,7 @@ struct tamx_split_plan {
   int64_t ntiles = 0;
   int32_t nslices = 0;
   int64_t limb_elems = 0;   // size of the limb pool in bf16 elements
-  int64_t nblocks = 0;      // 32 x 32 output blocks over all tiles
+  int64_t nblocks = 0;      // 64 x 32 blocks of the limb layouts over all tiles
   SplitTile* tiles = nullptr;        // device
   int64_t* block_prefix = nullptr;   // device, ntiles + 1
   unsigned long long* maxbits = nullptr;  // device, per-tile max |a| bits
@@ -67,7 +67,12 @@ struct tamx_split_plan {
 
 namespace {
 
-constexpr int kSplitTS = 32;
+// The split passes walk 64-row x 32-k blocks of every tile's limb layout;
+// thread t of a 256-thread workgroup owns row t / 4 and the 8 consecutive k
+// of unit t % 4 (one 16-byte store per limb plane).
+constexpr int kSplitBR = 64, kSplitBK = 32;
+
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t item) {
   int64_t lo = 0, hi = n - 1;
@@ -83,28 +88,61 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-// max |a| per tile over 32 x 32 blocks of the stored tile (the same block
-// enumeration as the split pass); positive doubles order like their bits
+// The 8 values of thread t's (row, 8 k) in block (r0, k0) of tile t_, zero
+// outside the tile.  A transposed tile (stored k x rows) is staged through
+// `stg` so the global reads run along the stored rows.
+__device__ inline void load_block8(const double* __restrict__ pool, const SplitTile& t, int r0,
+                                   int k0, double (*stg)[kSplitBR + 1], double (&v)[8]) {
+  const int tid = threadIdx.x;
+  const int r = tid >> 2, q = tid & 3;
+  if (t.transpose) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kk = (tid >> 6) + 4 * j, rr = tid & 63;
+      const int gk = k0 + kk, gr = r0 + rr;
+      double x = 0.0;
+      if (gk < t.out_k && gr < t.out_rows) x = pool[t.src_off + (int64_t)gk * t.cols + gr];
+      stg[kk][rr] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
+    __syncthreads();
+  } else {
+    const int gr = r0 + r;
+    const int64_t row = t.src_off + (int64_t)gr * t.cols;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int gk = k0 + 8 * q + i;
+      v[i] = (gr < t.out_rows && gk < t.out_k) ? pool[row + gk] : 0.0;
+    }
+  }
+}
+
+__device__ inline void block_of(const SplitTile& t, int64_t local, int& r0, int& k0) {
+  const int nbk = t.k_pad / kSplitBK;
+  r0 = (int)(local / nbk) * kSplitBR;
+  k0 = (int)(local % nbk) * kSplitBK;
+}
+
+// max |a| per tile; positive doubles order like their bits
 __global__ __launch_bounds__(256) void split_max_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
     const int64_t* __restrict__ prefix, int64_t ntiles,
     unsigned long long* __restrict__ maxbits) {
+  __shared__ double stg[kSplitBK][kSplitBR + 1];
   __shared__ double red[256];
   const int64_t nblocks = prefix[ntiles];
   for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
     const int64_t ti = find_tile(prefix, ntiles, item);
     const SplitTile& t = tiles[ti];
-    const int64_t local = item - prefix[ti];
-    const int nbk = t.k_pad / kSplitTS;
-    const int br = (int)(local / nbk), bk = (int)(local % nbk);
+    int r0, k0;
+    block_of(t, item - prefix[ti], r0, k0);
+    double v[8];
+    load_block8(pool, t, r0, k0, stg, v);
     double m = 0.0;
-    for (int e = threadIdx.x; e < kSplitTS * kSplitTS; e += 256) {
-      const int r = br * kSplitTS + e / kSplitTS, k = bk * kSplitTS + e % kSplitTS;
-      if (r < t.out_rows && k < t.out_k) {
-        const int64_t idx = t.transpose ? (int64_t)k * t.cols + r : (int64_t)r * t.cols + k;
-        m = fmax(m, fabs(pool[t.src_off + idx]));
-      }
-    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) m = fmax(m, fabs(v[i]));
     red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -117,79 +155,66 @@ __global__ __launch_bounds__(256) void split_max_kernel(
   }
 }
 
+// maxabs (when given) holds each tile's max |a|, else maxbits its bits
 __global__ void split_sigma_kernel(SplitTile* tiles, const unsigned long long* maxbits,
-                                   int64_t ntiles) {
+                                   const double* maxabs, int64_t ntiles) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  const double m = __longlong_as_double((long long)maxbits[t]);
+  const double m = maxabs ? maxabs[t] : __longlong_as_double((long long)maxbits[t]);
   // x = a * 2^-sigma lies in (-1, 1): m < 2^(ilogb(m) + 1)
   tiles[t].sigma = m > 0.0 ? ilogb(m) + 1 : 0;
 }
 
-// Writes the S limb planes of every 32 x 32 block of the limb layout.  A
-// transposed read goes through LDS so both the FP64 reads and the BF16
-// writes are contiguous.
+// Writes the S limb planes of every 64 x 32 block of the limb layout.
 template <int S>
 __global__ __launch_bounds__(256) void split_limbs_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
-    const int64_t* __restrict__ prefix, int64_t ntiles, __bf16* __restrict__ limbs) {
+    const int64_t* __restrict__ prefix, int64_t ntiles, bf8_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
-  __shared__ double tileb[kSplitTS][kSplitTS + 1];
+  __shared__ double stg[kSplitBK][kSplitBR + 1];
   const int64_t nblocks = prefix[ntiles];
-  const int tx = threadIdx.x % kSplitTS, ty = threadIdx.x / kSplitTS;  // 32 x 8
+  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
   for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
     const int64_t ti = find_tile(prefix, ntiles, item);
     const SplitTile& t = tiles[ti];
-    const int64_t local = item - prefix[ti];
-    const int nbk = t.k_pad / kSplitTS;
-    const int br = (int)(local / nbk), bk = (int)(local % nbk);
-    const int r0 = br * kSplitTS, k0 = bk * kSplitTS;
-    // stage the source block as tileb[row][k] of the limb layout
-    for (int y = ty; y < kSplitTS; y += 8) {
-      double v = 0.0;
-      if (t.transpose) {
-        // stored (k, row): read along row (contiguous), store transposed
-        const int k = k0 + y, r = r0 + tx;
-        if (r < t.out_rows && k < t.out_k) v = pool[t.src_off + (int64_t)k * t.cols + r];
-        tileb[tx][y] = v;
-      } else {
-        const int r = r0 + y, k = k0 + tx;
-        if (r < t.out_rows && k < t.out_k) v = pool[t.src_off + (int64_t)r * t.cols + k];
-        tileb[y][tx] = v;
-      }
-    }
-    __syncthreads();
-    const int64_t plane = (int64_t)t.rows_pad * t.k_pad;
-    for (int y = ty; y < kSplitTS; y += 8) {
-      const int r = r0 + y;
-      if (r >= t.rows_pad) continue;
-      const double x = ldexp(tileb[y][tx], -t.sigma);
-      const int64_t o = t.limb_off + (int64_t)r * t.k_pad + k0 + tx;
-      double xs = x * 256.0;
-      double l = rint(xs);
-      limbs[o] = (__bf16)(float)l;
-      double rem = xs - l;
+    int r0, k0;
+    block_of(t, item - prefix[ti], r0, k0);
+    double v[8];
+    load_block8(pool, t, r0, k0, stg, v);
+    if (r0 + r >= t.rows_pad) continue;
+    const int sigma = t.sigma;
+    bf8_t out[S];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double xs0 = ldexp(v[i], 8 - sigma);  // 256 * a * 2^-sigma
+      double l = rint(xs0);
+      out[0][i] = (__bf16)(float)l;
+      double rem = xs0 - l;
 #pragma unroll
       for (int s = 1; s < S; ++s) {
-        xs = rem * 512.0;
+        const double xs = rem * 512.0;
         l = rint(xs);
-        limbs[o + s * plane] = (__bf16)(float)ldexp(l, -9 * s);
+        out[s][i] = (__bf16)(float)ldexp(l, -9 * s);
         rem = xs - l;
       }
     }
-    __syncthreads();
+    // in units of 8 elements
+    const int64_t plane = ((int64_t)t.rows_pad * t.k_pad) >> 3;
+    const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
+#pragma unroll
+    for (int s = 0; s < S; ++s) limbs[o + s * plane] = out[s];
   }
 }
 
 template <int S>
 void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_plan* p,
-                  __bf16* limbs) {
-  split_limbs_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles, limbs);
+                  void* limbs) {
+  split_limbs_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles,
+                                             static_cast<bf8_t*>(limbs));
 }
 
 // ------------------------------------------------------------ the GEMM --
 
-typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -223,9 +248,11 @@ __device__ inline int64_t uni64(int64_t x) {
 }
 
 // LDS layout of one operand chunk: S planes x 64 rows x 32 k of BF16, as
-// 16-byte units (8 k each); unit q of row r sits at r * 4 + (q ^ (r & 3)) so
-// the 16 rows a quarter-wave reads fall in different banks.
-__device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ (r & 3)); }
+// 16-byte units (8 k each); unit q of row r sits at r * 4 + (q ^ ((r >> 1) & 3)),
+// so any 8 consecutive rows read at one unit cover all eight 16-byte slots
+// of a 128-byte bank window (and the 4 units of 2 rows that 8 staging lanes
+// write do too).
+__device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ ((r >> 1) & 3)); }
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
@@ -313,10 +340,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     int lp = t_pb, lk0 = 0, lkp = 0;
     int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0;
     auto set_pair = [&](int pi) {
-      const tamx_gemm_pair& pp = p.pairs[pi];
+      const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
       const int at = uni(pp.a_tile), bt = uni(pp.b_tile);
-      const SplitTile* A = p.ta + at;
-      const SplitTile* B = p.tb + bt;
+      const auto* A = tamx::scalar_ptr(p.ta) + at;
+      const auto* B = tamx::scalar_ptr(p.tb) + bt;
       lkp = uni(A->k_pad);
       // element offsets of this thread's staging row in plane 0, in units of 8
       la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> 3;
@@ -325,6 +352,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> 3;
     };
     if (lp < pend) set_pair(lp);
+    // Once the cursor has passed the last pair the loads repeat its first
+    // chunk, unused: every step issues the same loads, which keeps the wait
+    // counts before the LDS stores exact (a conditional load would make the
+    // compiler wait for all loads in flight).
     auto load_chunk = [&](bf8_t (&ra)[S], bf8_t (&rb)[S]) {
       const int64_t q = (lk0 >> 3) + st_q;
 #pragma unroll
@@ -332,11 +363,13 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         ra[s] = p.LA[la_row + s * la_plane + q];
         rb[s] = p.LB[lb_row + s * lb_plane + q];
       }
-      lk0 += SBK;
-      if (lk0 >= lkp) {
-        ++lp;
-        lk0 = 0;
-        if (lp < pend) set_pair(lp);
+      if (lp < pend) {
+        lk0 += SBK;
+        if (lk0 >= lkp) {
+          ++lp;
+          lk0 = 0;
+          if (lp < pend) set_pair(lp);
+        }
       }
     };
     auto store_chunk = [&](int buf, const bf8_t (&ra)[S], const bf8_t (&rb)[S]) {
@@ -382,8 +415,9 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       else if (nj == 2) compute_blocks(buf, one{}, two{});
       else compute_blocks(buf, one{}, one{});
     };
-    // fold the FP32 accumulators of the pair being computed (exponent e)
-    auto fold = [&](int e) {
+    // fold the FP32 accumulators of the pair being computed (scale
+    // 2^(sigma_A + sigma_B - 16))
+    auto fold = [&](double scale) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -396,18 +430,19 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
               tsum += (double)lvl[g][i][j][v];
               lvl[g][i][j][v] = 0.0f;
             }
-            acc[i][j][v] += ldexp(tsum, e - 16);
+            acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
           }
     };
 
     // compute cursor: pair cp, chunks done since the last fold
-    int cp = t_pb, ck0 = 0, ckp = 0, ce = 0, since_fold = 0;
+    int cp = t_pb, ck0 = 0, ckp = 0, since_fold = 0;
+    double cscale = 0.0;
     auto set_compute_pair = [&](int pi) {
-      const tamx_gemm_pair& pp = p.pairs[pi];
-      const SplitTile* A = p.ta + uni(pp.a_tile);
-      const SplitTile* B = p.tb + uni(pp.b_tile);
+      const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
+      const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
+      const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
       ckp = uni(A->k_pad);
-      ce = uni(A->sigma) + uni(B->sigma);
+      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - 16);
     };
     if (cp < pend) set_compute_pair(cp);
     // after computing one chunk: fold at a pair end or every kFoldChunks
@@ -416,7 +451,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       ck0 += SBK;
       const bool pair_end = ck0 >= ckp;
       if (pair_end || since_fold == kFoldChunks) {
-        if (wave_active) fold(ce);
+        if (wave_active) fold(cscale);
         since_fold = 0;
       }
       if (pair_end) {
@@ -439,7 +474,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         int buf = 0;
         for (;;) {
           const bool has_next = lp < pend;
-          if (has_next) load_chunk(ra0, rb0);
+          load_chunk(ra0, rb0);
           compute_chunk(buf);
           after_compute();
           if (!has_next) break;
@@ -451,13 +486,13 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     } else if (lp < pend) {
       load_chunk(ra0, rb0);                // chunk 0
       const bool have1 = lp < pend;
-      if (have1) load_chunk(ra1, rb1);     // chunk 1
+      load_chunk(ra1, rb1);                // chunk 1 (if have1)
       store_chunk(0, ra0, rb0);
       __syncthreads();
       bool more = have1;  // chunk c + 1 exists
       for (;;) {
         bool next2 = lp < pend;
-        if (next2) load_chunk(ra0, rb0);   // chunk c + 2 -> R0
+        load_chunk(ra0, rb0);              // chunk c + 2 -> R0 (if next2)
         compute_chunk(0);
         after_compute();
         if (!more) break;
@@ -465,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         __syncthreads();
         more = next2;
         next2 = lp < pend;
-        if (next2) load_chunk(ra1, rb1);   // chunk c + 2 -> R1
+        load_chunk(ra1, rb1);              // chunk c + 2 -> R1 (if next2)
         compute_chunk(1);
         after_compute();
         if (!more) break;
@@ -785,6 +820,43 @@ int launch_f32x2(tamx_context* ctx, tamx_worklist* wl, const double* A, const do
 
 }  // namespace
 
+namespace {
+
+int split_f64(tamx_context* ctx, const tamx_split_plan* plan, const double* pool,
+              const double* tile_maxabs, void* limbs, void* stream) {
+  TAMX_REQUIRE(plan != nullptr && limbs != nullptr, "null argument");
+  if (plan->ntiles == 0) return TAMX_OK;
+  TAMX_REQUIRE(pool != nullptr || plan->nblocks == 0, "null pool");
+  if (int rc = tamx::activate(ctx)) return rc;
+  hipStream_t s = tamx::as_stream(stream);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(plan->nblocks, ctx->num_cus * 16));
+  const unsigned tgrid = (unsigned)((plan->ntiles + 255) / 256);
+  if (!tile_maxabs) {
+    zero_u64_kernel<<<tgrid, 256, 0, s>>>(plan->maxbits, plan->ntiles);
+    TAMX_LAUNCH_CHECK();
+    if (plan->nblocks > 0) {
+      split_max_kernel<<<grid, 256, 0, s>>>(pool, plan->tiles, plan->block_prefix, plan->ntiles,
+                                            plan->maxbits);
+      TAMX_LAUNCH_CHECK();
+    }
+  }
+  split_sigma_kernel<<<tgrid, 256, 0, s>>>(plan->tiles, plan->maxbits, tile_maxabs, plan->ntiles);
+  TAMX_LAUNCH_CHECK();
+  if (plan->nblocks > 0) {
+    switch (plan->nslices) {
+      case 2: launch_split<2>(grid, s, pool, plan, limbs); break;
+      case 3: launch_split<3>(grid, s, pool, plan, limbs); break;
+      case 4: launch_split<4>(grid, s, pool, plan, limbs); break;
+      case 5: launch_split<5>(grid, s, pool, plan, limbs); break;
+      default: launch_split<6>(grid, s, pool, plan, limbs); break;
+    }
+    TAMX_LAUNCH_CHECK();
+  }
+  return TAMX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
@@ -825,7 +897,7 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
     st.limb_off = off;
     off += (int64_t)nslices * st.rows_pad * st.k_pad;
     max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
-    blocks += (int64_t)((st.rows_pad + kSplitTS - 1) / kSplitTS) * (st.k_pad / kSplitTS);
+    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * (st.k_pad / kSplitBK);
   }
   prefix[ntiles] = blocks;
   auto* plan = new (std::nothrow) tamx_split_plan();
@@ -883,34 +955,13 @@ int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma) {
 
 int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan, const double* pool,
                    void* limbs, void* stream) {
-  TAMX_REQUIRE(plan != nullptr && limbs != nullptr, "null argument");
-  if (plan->ntiles == 0) return TAMX_OK;
-  TAMX_REQUIRE(pool != nullptr || plan->nblocks == 0, "null pool");
-  if (int rc = tamx::activate(ctx)) return rc;
-  hipStream_t s = tamx::as_stream(stream);
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(plan->nblocks, ctx->num_cus * 16));
-  zero_u64_kernel<<<(unsigned)((plan->ntiles + 255) / 256), 256, 0, s>>>(plan->maxbits, plan->ntiles);
-  TAMX_LAUNCH_CHECK();
-  if (plan->nblocks > 0) {
-    split_max_kernel<<<grid, 256, 0, s>>>(pool, plan->tiles, plan->block_prefix, plan->ntiles,
-                                          plan->maxbits);
-    TAMX_LAUNCH_CHECK();
-  }
-  split_sigma_kernel<<<(unsigned)((plan->ntiles + 255) / 256), 256, 0, s>>>(
-      plan->tiles, plan->maxbits, plan->ntiles);
-  TAMX_LAUNCH_CHECK();
-  if (plan->nblocks > 0) {
-    __bf16* L = static_cast<__bf16*>(limbs);
-    switch (plan->nslices) {
-      case 2: launch_split<2>(grid, s, pool, plan, L); break;
-      case 3: launch_split<3>(grid, s, pool, plan, L); break;
-      case 4: launch_split<4>(grid, s, pool, plan, L); break;
-      case 5: launch_split<5>(grid, s, pool, plan, L); break;
-      default: launch_split<6>(grid, s, pool, plan, L); break;
-    }
-    TAMX_LAUNCH_CHECK();
-  }
-  return TAMX_OK;
+  return split_f64(ctx, plan, pool, nullptr, limbs, stream);
+}
+
+int tamx_split_f64_premax(tamx_context* ctx, const tamx_split_plan* plan, const double* pool,
+                          const double* tile_maxabs, void* limbs, void* stream) {
+  TAMX_REQUIRE(tile_maxabs != nullptr, "null tile_maxabs");
+  return split_f64(ctx, plan, pool, tile_maxabs, limbs, stream);
 }
 
 int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,

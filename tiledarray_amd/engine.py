@@ -210,14 +210,22 @@ class _SplitOperand:
 
 # ------------------------------------------------------------- shapes ------
 
-def tile_norms(ctx, arr, out=None):
-    """Raw per-tile Frobenius norms (float32, device), Tensor::norm()."""
+def tile_norms(ctx, arr, out=None, maxabs=None):
+    """Raw per-tile Frobenius norms (float32, device), Tensor::norm().  With
+    `maxabs` (float64, one per tile; FP64 arrays only) the same pass also
+    writes each tile's max |a|, the scale input of the limb split."""
     plan = getattr(arr, "_norm_plan", None)
     if plan is None or plan.ctx is not ctx:
         plan = NormPlan(ctx, arr.offsets, arr.trange.tile_volumes())
         arr._norm_plan = plan
     if out is None:
         out = torch.empty(arr.trange.ntiles, dtype=torch.float32, device=arr.pool.device)
+    if maxabs is not None:
+        if arr.dtype != torch.float64 or maxabs.dtype != torch.float64:
+            raise ValueError("tile maxima are produced for FP64 arrays into float64 tensors")
+        call("tamx_tile_norms_max_f64", ctx.ptr, plan.ptr, arr.pool.data_ptr(), out.data_ptr(),
+             maxabs.data_ptr(), ctx.stream())
+        return out
     f = "tamx_tile_norms_f64" if arr.dtype == torch.float64 else "tamx_tile_norms_f32"
     call(f, ctx.ptr, plan.ptr, arr.pool.data_ptr(), out.data_ptr(), ctx.stream())
     return out
@@ -399,14 +407,21 @@ class Contraction:
              int(op), offs.size, offs.ctypes.data, rows.ctypes.data, cols.ctypes.data,
              ctypes.byref(p))
         n = int(lib().tamx_split_plan_limb_elems(p))
-        limbs = torch.empty(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
+        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
         return _SplitOperand(p, limbs, offs.size)
 
-    def split_step(self):
-        """Split both operands into BF16 limbs (part of the split path)."""
-        for X, sp in ((self.A, self.a_split), (self.B, self.b_split)):
-            call("tamx_split_f64", self.ctx.ptr, sp.ptr, X.pool.data_ptr(), sp.limbs.data_ptr(),
-                 self.ctx.stream())
+    def split_step(self, maxabs=None):
+        """Split both operands into BF16 limbs (part of the split path).
+        `maxabs`: optional (A, B) per-tile max |a| tensors (tile_norms with
+        maxabs, by the operands' tile ordinals) that spare the max pass."""
+        for w, (X, sp) in enumerate(((self.A, self.a_split), (self.B, self.b_split))):
+            m = None if maxabs is None else maxabs[w]
+            if m is None:
+                call("tamx_split_f64", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
+                     sp.limbs.data_ptr(), self.ctx.stream())
+            else:
+                call("tamx_split_f64_premax", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
+                     m.data_ptr(), sp.limbs.data_ptr(), self.ctx.stream())
 
     def sigmas(self):
         out = []
