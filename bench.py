@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--no-split", action="store_true", help="skip the split-precision runs")
     p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,f32x2",
                    help="split precisions measured beside the FP64 run")
+    p.add_argument("--split-tile", default=None,
+                   help="macro tile BMxBN of the split GEMM (64x64, 128x64 or 64x128)")
     return p.parse_args()
 
 
@@ -271,7 +273,10 @@ def main():
         rn, rmax = float(np.linalg.norm(r)), float(np.abs(r).max())
         split = {}
         for prec in args.split.split(","):
-            cs = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, precision=prec)
+            st = (tuple(int(x) for x in args.split_tile.split("x"))
+                  if args.split_tile and prec.startswith("bf16") and int(prec[5:]) <= 4 else None)
+            cs = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, precision=prec,
+                               tile=st)
             ms_s, gemm_s = time_steps(cs, make_step(cs), args.steps, args.warmup)
             g = cs.C.cpu().numpy()[used]
             split[prec] = {
@@ -279,6 +284,7 @@ def main():
                 "ms_per_step": round(ms_s, 4),
                 "gemm_tflops": round(flops / (gemm_s * 1e-3) / 1e12, 3),
                 "speedup_vs_f64": round(ms / ms_s, 3),
+                "macro_tile": list(st) if st else [64, 64],
                 "rel_err_frobenius": float(np.linalg.norm(g - r) / rn),
                 "rel_err_max": float(np.abs(g - r).max() / rmax),
                 "error_against": "oracle" if ref is not None else "fp64 result",

@@ -83,6 +83,47 @@ def test_norms_with_max_and_premax_split(ctx):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
+@pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
+@pytest.mark.parametrize("nslices", [2, 3, 4])
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+def test_split_wide_tiles_integer_exact(ctx, tile, nslices, op_a, op_b):
+    # the wide macro tiles (one wave per SIMD) on tiles wider and narrower
+    # than one macro tile, ragged in both directions
+    rng = np.random.default_rng(900 + nslices * 4 + op_a * 2 + op_b)
+    trm = TiledRange1([0, 40, 170, 300, 301])
+    trk = TiledRange1([0, 17, 64, 150, 151, 300])
+    trn = TiledRange1([0, 96, 97, 250])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
+    la = "k,m" if op_a else "m,k"
+    lb = "n,k" if op_b else "k,n"
+    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=f"bf16x{nslices}",
+                      tile=tile)
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+@pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
+def test_split_wide_tiles_match_default(ctx, tile):
+    # same limbs, same pair order: the wide kernel's result equals the
+    # default kernel's bit for bit on random data
+    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=21)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="bf16x4")
+    c.run()
+    torch.cuda.synchronize()
+    C0 = c.C.clone()
+    c.set_tile(tile)
+    c.C.zero_()
+    c.gemm_step()
+    torch.cuda.synchronize()
+    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
+    assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
+
+
 def test_split_sigma_is_tile_exponent(ctx):
     rng = np.random.default_rng(5)
     tr = TiledRange1([0, 20, 45])

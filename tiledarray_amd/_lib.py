@@ -134,6 +134,7 @@ SIGNATURES = {
     "tamx_split_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_split": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_f32x2": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_split_config": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_autotune_grouped_gemm": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp],
     "tamx_autotune_clear": [],
     "tamx_graph_begin": [_vp, _vp],

@@ -37,42 +37,17 @@
 
 #include "tamx_internal.hpp"
 #include "tamx_scan.hpp"
+#include "tamx_split.hpp"
 
 namespace {
 
-struct SplitTile {
-  int64_t src_off;   // element offset of the FP64 tile
-  int64_t limb_off;  // element offset of limb plane 0 in the limb pool
-  int32_t rows, cols;  // stored FP64 tile (row-major)
-  int32_t out_rows;    // logical rows of the limb layout (m for A, n for B)
-  int32_t out_k;       // logical K
-  int32_t rows_pad;    // out_rows rounded up to 16
-  int32_t k_pad;       // out_k rounded up to 32 (row stride of a plane)
-  int32_t transpose;   // limb (row, k) reads stored (k, row)
-  int32_t sigma;       // scale exponent, written by the max pass
-};
-
-}  // namespace
-
-struct tamx_split_plan {
-  tamx_context* ctx = nullptr;
-  int64_t ntiles = 0;
-  int32_t nslices = 0;
-  int64_t limb_elems = 0;   // size of the limb pool in bf16 elements
-  int64_t nblocks = 0;      // 64 x 32 blocks of the limb layouts over all tiles
-  SplitTile* tiles = nullptr;        // device
-  int64_t* block_prefix = nullptr;   // device, ntiles + 1
-  unsigned long long* maxbits = nullptr;  // device, per-tile max |a| bits
-};
-
-namespace {
+using namespace tamx::split;
+constexpr int SBM = 64, SBN = 64;  // macro tile of split_gemm_kernel
 
 // The split passes walk 64-row x 32-k blocks of every tile's limb layout;
 // thread t of a 256-thread workgroup owns row t / 4 and the 8 consecutive k
 // of unit t % 4 (one 16-byte store per limb plane).
 constexpr int kSplitBR = 64, kSplitBK = 32;
-
-typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t item) {
   int64_t lo = 0, hi = n - 1;
@@ -214,45 +189,6 @@ void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_
 }
 
 // ------------------------------------------------------------ the GEMM --
-
-typedef float f4_t __attribute__((ext_vector_type(4)));
-typedef double d4_t __attribute__((ext_vector_type(4)));
-
-struct SplitParams {
-  const bf8_t* __restrict__ LA;  // limb pools viewed as 8-element vectors
-  const bf8_t* __restrict__ LB;
-  const SplitTile* __restrict__ ta;
-  const SplitTile* __restrict__ tb;
-  double* __restrict__ C;
-  const tamx_gemm_task* __restrict__ tasks;
-  const tamx_gemm_pair* __restrict__ pairs;
-  const int64_t* __restrict__ counters;
-  const int32_t* __restrict__ prefix;
-  const int32_t* __restrict__ item_task;
-  int32_t* queue;
-  int64_t task_capacity;
-  int64_t item_capacity;
-  double alpha;
-};
-
-constexpr int kThreads = 256;
-constexpr int kRegions = 8;
-constexpr int SBM = 64, SBN = 64, SBK = 32;
-constexpr int kFoldChunks = 4;  // fold FP32 levels into FP64 every 128 k
-
-__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ inline int64_t uni64(int64_t x) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// LDS layout of one operand chunk: S planes x 64 rows x 32 k of BF16, as
-// 16-byte units (8 k each); unit q of row r sits at r * 4 + (q ^ ((r >> 1) & 3)),
-// so any 8 consecutive rows read at one unit cover all eight 16-byte slots
-// of a 128-byte bank window (and the 4 units of 2 rows that 8 staging lanes
-// write do too).
-__device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ ((r >> 1) & 3)); }
 
 template <int S>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
@@ -905,9 +841,7 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
   plan->ctx = ctx;
   plan->ntiles = ntiles;
   plan->nslices = nslices;
-  // slack past the last tile: a 64-row macro tile may read up to 63 rows
-  // beyond a tile's padded rows (those rows only reach unstored results)
-  plan->limb_elems = off + 64 * max_kpad * nslices;
+  plan->limb_elems = off + (int64_t)kLimbSlackRows * max_kpad * nslices;
   plan->nblocks = blocks;
   hipError_t e = hipMalloc(&plan->tiles, std::max<size_t>(1, tiles.size()) * sizeof(SplitTile));
   if (e == hipSuccess) e = hipMalloc(&plan->block_prefix, prefix.size() * sizeof(int64_t));
@@ -968,6 +902,14 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             const tamx_split_plan* a_plan, const void* a_limbs,
                             const tamx_split_plan* b_plan, const void* b_limbs,
                             double* C, void* stream) {
+  return tamx_grouped_gemm_split_config(ctx, wl, alpha, a_plan, a_limbs, b_plan, b_limbs, C,
+                                        nullptr, stream);
+}
+
+int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double alpha,
+                                   const tamx_split_plan* a_plan, const void* a_limbs,
+                                   const tamx_split_plan* b_plan, const void* b_limbs,
+                                   double* C, const tamx_gemm_config* cfg, void* stream) {
   TAMX_REQUIRE(wl && a_plan && b_plan && a_limbs && b_limbs, "null argument");
   TAMX_REQUIRE(a_plan->nslices == b_plan->nslices, "operands split into different limb counts");
   if (int rc = tamx::activate(ctx)) return rc;
@@ -986,6 +928,8 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                   wl->item_capacity,
                   alpha};
   hipStream_t s = tamx::as_stream(stream);
+  if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
+    return launch_split_gemm_wide(a_plan->nslices, cfg->bm, cfg->bn, ctx, wl, prm, s);
   switch (a_plan->nslices) {
     case 2: return launch_split_gemm<2>(ctx, wl, prm, s);
     case 3: return launch_split_gemm<3>(ctx, wl, prm, s);

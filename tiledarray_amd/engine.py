@@ -504,9 +504,9 @@ class Contraction:
             call("tamx_grouped_gemm_f32x2", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
                  ctypes.c_double(self.alpha), A, B, C, s)
         else:
-            call("tamx_grouped_gemm_split", self.ctx.ptr, self.wl.ptr, ctypes.c_double(self.alpha),
-                 self.a_split.ptr, self.a_split.limbs.data_ptr(), self.b_split.ptr,
-                 self.b_split.limbs.data_ptr(), C, s)
+            call("tamx_grouped_gemm_split_config", self.ctx.ptr, self.wl.ptr,
+                 ctypes.c_double(self.alpha), self.a_split.ptr, self.a_split.limbs.data_ptr(),
+                 self.b_split.ptr, self.b_split.limbs.data_ptr(), C, cfg, s)
 
     def run(self):
         if self.permute_operands:
