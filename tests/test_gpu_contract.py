@@ -208,6 +208,21 @@ def test_permute_bit_exact_vs_oracle(ctx, rank):
         np.testing.assert_array_equal(Y.to_dense(), oracle.permute(X.to_dense(), perm))
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_permute_large_and_mixed_planes(ctx, dtype):
+    # planes above 96 x 96 go through 32 x 32 tiles, smaller ones whole; one
+    # plan holds both kinds, with ragged edges on every tile
+    rng = np.random.default_rng(77)
+    tr = TiledRange([TiledRange1([0, 130, 170, 301]), TiledRange1([0, 97, 100, 260]),
+                     TiledRange1([0, 3, 5])])
+    X = DistArray.allocate(tr, rng.random(tr.ntiles) < 0.9, dtype=dtype)
+    X.pool.copy_(torch.as_tensor(rng.uniform(-1, 1, X.pool.numel()), dtype=dtype))
+    for perm in ([1, 0, 2], [2, 1, 0], [0, 2, 1], [1, 2, 0]):
+        Y = E.permute_array(ctx, X, perm)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(Y.to_dense(), oracle.permute(X.to_dense(), perm))
+
+
 def test_ladder_permutes_and_result_transpose(ctx):
     # R(a,b,i,j) = T(c,d,i,j) * V(a,c,b,d): V is permuted (general), T folds
     # into Trans, and the result permutation is fused into the GEMM store

@@ -5,9 +5,10 @@
 // fused first, as detail::permute does (tensor/permute.h:51-95).  Two cases:
 //   - the fastest source mode stays fastest: contiguous runs are copied;
 //   - otherwise the (fastest source mode) x (mode that becomes fastest in the
-//     result) plane is moved through a padded 32 x 32 LDS tile so that both
-//     the loads and the stores are contiguous along a row; every other mode
-//     is a batch index.
+//     result) plane is moved through LDS so that both the loads and the
+//     stores are contiguous along a row: whole when it has at most 96 x 96
+//     elements, else in padded 32 x 32 tiles; every other mode is a batch
+//     index.
 // Both are HBM-bandwidth bound: 2 * volume * sizeof(T) bytes per tile.
 #include <algorithm>
 #include <new>
@@ -20,6 +21,11 @@ namespace {
 constexpr int kMaxRank = 8;
 constexpr int kTS = 32;       // transpose tile edge
 constexpr int kCopyRun = 2048;  // elements per copy item
+// a transposed plane of at most this many elements moves through LDS whole
+// (one item per plane, no partial edge tiles)
+constexpr int kPlaneMax = 96 * 96;
+constexpr int kPermThreads = 512;
+constexpr int kRowsPer = kPermThreads / kTS;  // tile rows a pass of the workgroup covers
 
 struct PermTile {
   int64_t src_off, dst_off;
@@ -30,7 +36,7 @@ struct PermTile {
   int32_t q;      // fused rank
   int32_t s_dim;  // fastest source mode (q - 1)
   int32_t t_dim;  // source mode that is fastest in the result
-  int32_t nbs, nbt;  // blocks along s and t
+  int32_t nbs, nbt;  // blocks along s and t (0 for a whole-plane tile)
 };
 
 }  // namespace
@@ -55,44 +61,22 @@ __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t it
   return lo;
 }
 
+// `plane` is the workgroup's LDS staging area (a whole plane, or one padded
+// 32 x 32 tile at its start).
 template <typename T>
 __device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ dst,
-                                    const PermTile* __restrict__ tiles,
-                                    const int64_t* __restrict__ prefix, int64_t ntiles,
-                                    int64_t item, T (*lds)[kTS + 1]);
-
-template <typename T>
-__global__ __launch_bounds__(256) void permute_kernel(
-    const T* __restrict__ src, T* __restrict__ dst,
-    const PermTile* __restrict__ tiles, const int64_t* __restrict__ prefix,
-    int64_t ntiles) {
-  __shared__ T lds[kTS][kTS + 1];
-  const int64_t nitems = prefix[ntiles];
-  for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
-    permute_item<T>(src, dst, tiles, prefix, ntiles, item, lds);
-    __syncthreads();
-  }
-}
-
-template <typename T>
-__device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ dst,
-                                    const PermTile* __restrict__ tiles,
-                                    const int64_t* __restrict__ prefix, int64_t ntiles,
-                                    int64_t item, T (*lds)[kTS + 1]) {
-  const int64_t ti = find_tile(prefix, ntiles, item);
-  const PermTile& t = tiles[ti];
-  int64_t local = item - prefix[ti];
+                                    const PermTile& t, int local, T* plane) {
   const int q = t.q, sd = t.s_dim, td = t.t_dim;
 
   if (sd == td) {
     // contiguous-run copy: item = (other modes, run block)
     const int nruns = (t.ext[sd] + kCopyRun - 1) / kCopyRun;
-    const int rb = (int)(local % nruns);
+    const int rb = local % nruns;
     local /= nruns;
     int64_t so = t.src_off, doff = t.dst_off;
     for (int d = q - 1; d >= 0; --d) {
       if (d == sd) continue;
-      const int64_t x = local % t.ext[d];
+      const int x = local % t.ext[d];
       local /= t.ext[d];
       so += x * t.sstride[d];
       doff += x * t.dstride[d];
@@ -103,29 +87,93 @@ __device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ d
     return;
   }
 
-  const int bs = (int)(local % t.nbs);
+  if (t.nbs == 0) {
+    // whole plane: ns x nt elements, LDS laid out [t][s] with odd stride
+    const int ns = t.ext[sd], nt = t.ext[td];
+    int64_t so = t.src_off, doff = t.dst_off;
+    for (int d = q - 1; d >= 0; --d) {
+      if (d == sd || d == td) continue;
+      const int x = local % t.ext[d];
+      local /= t.ext[d];
+      so += x * t.sstride[d];
+      doff += x * t.dstride[d];
+    }
+    T* pl = plane;
+    const int ld = ns | 1;
+    const int n = ns * nt;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const int y = e / ns, x = e - y * ns;  // t index, s index
+      pl[y * ld + x] = src[so + (int64_t)y * t.sstride[td] + x];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const int y = e / nt, x = e - y * nt;  // s index, t index
+      dst[doff + (int64_t)y * t.dstride[sd] + x] = pl[x * ld + y];
+    }
+    return;
+  }
+
+  const int bs = local % t.nbs;
   local /= t.nbs;
-  const int bt = (int)(local % t.nbt);
+  const int bt = local % t.nbt;
   local /= t.nbt;
   int64_t so = t.src_off, doff = t.dst_off;
   for (int d = q - 1; d >= 0; --d) {
     if (d == sd || d == td) continue;
-    const int64_t x = local % t.ext[d];
+    const int x = local % t.ext[d];
     local /= t.ext[d];
     so += x * t.sstride[d];
     doff += x * t.dstride[d];
   }
   const int s0 = bs * kTS, t0 = bt * kTS;
   const int ns = min(kTS, t.ext[sd] - s0), nt = min(kTS, t.ext[td] - t0);
-  const int tx = threadIdx.x % kTS, ty = threadIdx.x / kTS;  // 32 x 8
-  // load: rows along t, contiguous along s (source stride 1)
-  for (int y = ty; y < kTS; y += 256 / kTS)
-    if (tx < ns && y < nt) lds[y][tx] = src[so + (int64_t)(t0 + y) * t.sstride[td] + s0 + tx];
+  const int tx = threadIdx.x % kTS, ty = threadIdx.x / kTS;  // 32 x kRowsPer
+  T (*lds)[kTS + 1] = reinterpret_cast<T (*)[kTS + 1]>(plane);
+  // load: rows along t, contiguous along s (source stride 1); all of a
+  // thread's loads are issued before the LDS writes
+  T v[kTS / kRowsPer];
+#pragma unroll
+  for (int k = 0; k < kTS / kRowsPer; ++k) {
+    const int y = ty + kRowsPer * k;
+    v[k] = (tx < ns && y < nt) ? src[so + (int64_t)(t0 + y) * t.sstride[td] + s0 + tx] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kTS / kRowsPer; ++k) lds[ty + kRowsPer * k][tx] = v[k];
   __syncthreads();
   // store: rows along s, contiguous along t (result stride 1); the caller's
   // barrier after the item protects the LDS tile before its next reuse
-  for (int y = ty; y < kTS; y += 256 / kTS)
+#pragma unroll
+  for (int k = 0; k < kTS / kRowsPer; ++k) {
+    const int y = ty + kRowsPer * k;
     if (tx < nt && y < ns) dst[doff + (int64_t)(s0 + y) * t.dstride[sd] + t0 + tx] = lds[tx][y];
+  }
+}
+
+// Each workgroup takes a contiguous range of items, so consecutive items
+// mostly belong to one tile: the tile is found by binary search once and
+// then followed forward.
+template <typename T>
+__global__ __launch_bounds__(kPermThreads) void permute_kernel(
+    const T* __restrict__ src, T* __restrict__ dst,
+    const PermTile* __restrict__ tiles, const int64_t* __restrict__ prefix,
+    int64_t ntiles) {
+  static_assert(kPlaneMax + 96 >= kTS * (kTS + 1), "staging area holds a tile");
+  __shared__ T plane[kPlaneMax + 96];  // rows padded to an odd length
+  const int64_t nitems = prefix[ntiles];
+  const int64_t per = (nitems + gridDim.x - 1) / gridDim.x;
+  const int64_t ib = min<int64_t>(nitems, (int64_t)blockIdx.x * per);
+  const int64_t ie = min<int64_t>(nitems, ib + per);
+  if (ib >= ie) return;
+  int64_t ti = find_tile(prefix, ntiles, ib);
+  int64_t tnext = prefix[ti + 1];
+  for (int64_t item = ib; item < ie; ++item) {
+    while (item >= tnext) {
+      ++ti;
+      tnext = prefix[ti + 1];
+    }
+    permute_item<T>(src, dst, tiles[ti], (int)(item - prefix[ti]), plane);
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -234,9 +282,11 @@ int tamx_permute_plan_create(tamx_context* ctx, int rank, const int32_t* perm,
       for (int d = 0; d < q; ++d)
         if (d != t.s_dim) nitems *= t.ext[d];
     } else {
-      t.nbs = (t.ext[t.s_dim] + kTS - 1) / kTS;
-      t.nbt = (t.ext[t.t_dim] + kTS - 1) / kTS;
-      nitems = (int64_t)t.nbs * t.nbt;
+      const bool whole = (int64_t)(t.ext[t.s_dim] | 1) * t.ext[t.t_dim] <= kPlaneMax + 96 &&
+                         t.ext[t.s_dim] <= 96 && t.ext[t.t_dim] <= 96;
+      t.nbs = whole ? 0 : (t.ext[t.s_dim] + kTS - 1) / kTS;
+      t.nbt = whole ? 0 : (t.ext[t.t_dim] + kTS - 1) / kTS;
+      nitems = whole ? 1 : (int64_t)t.nbs * t.nbt;
       for (int d = 0; d < q; ++d)
         if (d != t.s_dim && d != t.t_dim) nitems *= t.ext[d];
     }
@@ -291,7 +341,7 @@ int tamx_permute_f64(tamx_context* ctx, const tamx_permute_plan* plan,
   TAMX_REQUIRE(src != nullptr && dst != nullptr && src != dst,
                "src and dst must be distinct non-null pools");
   if (int rc = tamx::activate(ctx)) return rc;
-  permute_kernel<double><<<permute_grid(ctx, plan), 256, 0, tamx::as_stream(stream)>>>(
+  permute_kernel<double><<<permute_grid(ctx, plan), kPermThreads, 0, tamx::as_stream(stream)>>>(
       src, dst, plan->tiles, plan->item_prefix, plan->ntiles);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
@@ -304,7 +354,7 @@ int tamx_permute_f32(tamx_context* ctx, const tamx_permute_plan* plan,
   TAMX_REQUIRE(src != nullptr && dst != nullptr && src != dst,
                "src and dst must be distinct non-null pools");
   if (int rc = tamx::activate(ctx)) return rc;
-  permute_kernel<float><<<permute_grid(ctx, plan), 256, 0, tamx::as_stream(stream)>>>(
+  permute_kernel<float><<<permute_grid(ctx, plan), kPermThreads, 0, tamx::as_stream(stream)>>>(
       src, dst, plan->tiles, plan->item_prefix, plan->ntiles);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
