@@ -32,7 +32,7 @@ PEAK_FP32_TFLOPS = 157.3
 # of 1 GiB (524302 FETCH_SIZE units read, 1048576 WRITE_SIZE units written:
 # 2 KiB and 1 KiB per unit).  profiles/ holds the summaries.  Keyed by
 # workload, precision and macro tile.
-TRAFFIC_BYTES = {("blocksparse", "f64", (64, 64, 16)): 3044494 * 2048 + 925737 * 1024}
+TRAFFIC_BYTES = {("blocksparse", "f64", (64, 64, 16)): 3785074 * 2048 + 942487 * 1024}
 
 
 def parse():
