@@ -213,7 +213,8 @@ def main():
 
     c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, tile=tile,
                       precision=args.precision)
-    used_tile = autotune(args, c, make_step(c)) if args.precision in ("f64", "f32") else None
+    used_tile = (autotune(args, c, make_step(c))
+                 if args.precision in ("f64", "f32") or args.precision.startswith("bf16") else None)
     ms, gemm_ms = time_steps(c, make_step(c), args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
@@ -277,6 +278,10 @@ def main():
                   if args.split_tile and prec.startswith("bf16") and int(prec[5:]) <= 4 else None)
             cs = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, precision=prec,
                                tile=st)
+            if st is None and args.tile == "auto" and prec.startswith("bf16"):
+                make_step(cs)(None)
+                torch.cuda.synchronize()
+                st = tuple(cs.tune(reps=2)[0][:2])
             ms_s, gemm_s = time_steps(cs, make_step(cs), args.steps, args.warmup)
             g = cs.C.cpu().numpy()[used]
             split[prec] = {
@@ -356,7 +361,8 @@ def bench_ladder(args):
         else:
             c.gemm_step()
 
-    used_tile = autotune(args, c, step, reps=1) if args.precision == "f64" else None
+    used_tile = (autotune(args, c, step, reps=1)
+                 if args.precision == "f64" or args.precision.startswith("bf16") else None)
     ms, gemm_ms = time_steps(c, step, args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
@@ -368,6 +374,7 @@ def bench_ladder(args):
                     {"v_tiles": info["v_tiles"], "result_tiles": cnt["ntasks"],
                      "macro_tile": used_tile or ([64, 64, 16] if args.precision == "f64"
                                                  else [64, 64, 32]),
+                     "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
                      "precision": args.precision, "seed": info["seed"]},
                     {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),

@@ -338,6 +338,8 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
                            const int32_t* host_rows, const int32_t* host_cols,
                            tamx_split_plan** plan);
 int tamx_split_plan_destroy(tamx_split_plan* plan);
+/* Limb count of a plan (0 for a null plan). */
+int tamx_split_plan_nslices(const tamx_split_plan* plan);
 /* BF16 elements the limb pool of a plan needs. */
 int64_t tamx_split_plan_limb_elems(const tamx_split_plan* plan);
 /* Copies the per-tile scale exponents to the host (synchronous). */
@@ -401,6 +403,14 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl,
                                const void* A, const void* B, const void* C,
                                int reps, int flags, tamx_gemm_config* best,
                                float* best_ms, void* stream);
+/* The same for the split GEMM (tamx_grouped_gemm_split_config): times its
+ * macro tiles (64x64; up to four limbs also 128x64 and 64x128) and returns
+ * the fastest; cached per device, limb count and shape class. */
+int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
+                                     const tamx_split_plan* a_plan, const void* a_limbs,
+                                     const tamx_split_plan* b_plan, const void* b_limbs,
+                                     const double* C, int reps, int flags,
+                                     tamx_gemm_config* best, float* best_ms, void* stream);
 /* Drops every cached tuning result of the process. */
 int tamx_autotune_clear(void);
 

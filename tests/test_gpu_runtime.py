@@ -1,6 +1,6 @@
 """GPU tests of the runtime services: the tile-configuration autotuner and
 graph capture through the C ABI (tamx_autotune_grouped_gemm,
-tamx_graph_begin/_end/_launch).
+tamx_autotune_grouped_gemm_split, tamx_graph_begin/_end/_launch).
 
 Bars: the autotuner returns a compiled configuration and leaves the
 caller's C bit-for-bit unchanged; a contraction run with the tuned
@@ -93,6 +93,23 @@ def test_autotune_empty_worklist_returns_default(ctx):
     tile, ms = E.autotune(ctx, wl, torch.float32, 0, 0, x, x, x, reps=1, force=True)
     g0 = _lib.gemm_configs()[0]
     assert tile[:3] == (g0.bm, g0.bn, g0.bk) and ms == 0.0
+
+
+def test_autotune_split_picks_a_split_tile_with_identical_results(ctx):
+    # the split GEMM's tuner times its macro tiles; every one gives the same
+    # result bit for bit (same limbs, same pair and MFMA order)
+    A, B, _ = blocksparse_integer(ctx, n=1500, seed=31)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="bf16x4")
+    c.run()
+    torch.cuda.synchronize()
+    ref = c.C.clone()
+    tile, ms = c.tune(reps=1, force=True)
+    assert tile[:2] in ((64, 64), (128, 64), (64, 128)) and ms > 0.0
+    c.C.zero_()
+    c.gemm_step()
+    torch.cuda.synchronize()
+    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device=c.C.device)
+    assert torch.equal(c.C[:mask.numel()][mask], ref[:mask.numel()][mask])
 
 
 def test_autotune_rejects_split_precision(ctx):

@@ -137,6 +137,9 @@ SIGNATURES = {
     "tamx_grouped_gemm_split_config": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_autotune_grouped_gemm": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp],
     "tamx_autotune_clear": [],
+    "tamx_autotune_grouped_gemm_split": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
+                                         _vp],
+    "tamx_split_plan_nslices": [_vp],
     "tamx_graph_begin": [_vp, _vp],
     "tamx_graph_end": [_vp, _vp, _vp],
     "tamx_graph_launch": [_vp, _vp],

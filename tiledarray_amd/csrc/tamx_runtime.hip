@@ -189,6 +189,89 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int precisi
   return TAMX_OK;
 }
 
+int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
+                                     const tamx_split_plan* a_plan, const void* a_limbs,
+                                     const tamx_split_plan* b_plan, const void* b_limbs,
+                                     const double* C, int reps, int flags,
+                                     tamx_gemm_config* best, float* best_ms, void* stream) {
+  TAMX_REQUIRE(wl && a_plan && b_plan && a_limbs && b_limbs && best, "null argument");
+  TAMX_REQUIRE(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+  if (int rc = tamx::activate(ctx)) return rc;
+  hipStream_t s = tamx::as_stream(stream);
+  WorkListSummary sum;
+  if (int rc = summarize(wl, s, sum)) return rc;
+  const int nslices = tamx_split_plan_nslices(a_plan);
+  // the split GEMM's macro tiles: 64 x 64 for any limb count, 128 x 64 and
+  // 64 x 128 up to four limbs
+  std::vector<tamx_gemm_config> cands{{64, 64, 32, 0}};
+  if (nslices <= 4) {
+    cands.push_back({128, 64, 32, 0});
+    cands.push_back({64, 128, 32, 0});
+  }
+  *best = cands[0];
+  if (best_ms) *best_ms = 0.0f;
+  if (sum.ntasks == 0) return TAMX_OK;
+  TAMX_REQUIRE(C != nullptr || !sum.accumulates, "null C");
+  const TuneKey key{ctx->device, 100 + nslices, 0, bucket(sum.mean_m), bucket(sum.mean_n),
+                    bucket(sum.mean_k), bucket((double)sum.ntasks), 0};
+  if (!(flags & TAMX_TUNE_FORCE)) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *best = it->second.cfg;
+      if (best_ms) *best_ms = it->second.ms;
+      return TAMX_OK;
+    }
+  }
+  const size_t cbytes = (size_t)std::max<int64_t>(sum.c_extent, 1) * sizeof(double);
+  HipBuffer tmp;
+  HipEvent e0, e1;
+  TAMX_CHECK_HIP(hipMalloc(&tmp.p, cbytes));
+  TAMX_CHECK_HIP(hipEventCreate(&e0.e));
+  TAMX_CHECK_HIP(hipEventCreate(&e1.e));
+  auto seed = [&]() -> int {
+    if (sum.accumulates)
+      TAMX_CHECK_HIP(hipMemcpyAsync(tmp.p, C, cbytes, hipMemcpyDeviceToDevice, s));
+    return TAMX_OK;
+  };
+  auto launch = [&](const tamx_gemm_config& cfg) -> int {
+    return tamx_grouped_gemm_split_config(ctx, wl, 1.0, a_plan, a_limbs, b_plan, b_limbs,
+                                          static_cast<double*>(tmp.p), &cfg, s);
+  };
+  float best_t = -1.0f;
+  tamx_gemm_config win = cands[0];
+  for (const auto& cfg : cands) {
+    if (!(flags & TAMX_TUNE_NO_WARMUP)) {
+      if (int rc = seed()) return rc;
+      if (int rc = launch(cfg)) return rc;
+    }
+    float total = 0.0f;
+    for (int r = 0; r < reps; ++r) {
+      if (int rc = seed()) return rc;
+      TAMX_CHECK_HIP(hipEventRecord(e0.e, s));
+      if (int rc = launch(cfg)) return rc;
+      TAMX_CHECK_HIP(hipEventRecord(e1.e, s));
+      TAMX_CHECK_HIP(hipEventSynchronize(e1.e));
+      float ms = 0.0f;
+      TAMX_CHECK_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+      total += ms;
+    }
+    const float mean = total / reps;
+    if (best_t < 0.0f || mean < best_t) {
+      best_t = mean;
+      win = cfg;
+    }
+  }
+  TAMX_CHECK_HIP(hipStreamSynchronize(s));
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[key] = TuneEntry{win, best_t};
+  }
+  *best = win;
+  if (best_ms) *best_ms = best_t;
+  return TAMX_OK;
+}
+
 int tamx_autotune_clear(void) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tune_cache.clear();

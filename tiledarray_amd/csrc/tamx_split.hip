@@ -873,6 +873,10 @@ int tamx_split_plan_destroy(tamx_split_plan* plan) {
   return TAMX_OK;
 }
 
+int tamx_split_plan_nslices(const tamx_split_plan* plan) {
+  return plan ? plan->nslices : 0;
+}
+
 int64_t tamx_split_plan_limb_elems(const tamx_split_plan* plan) {
   return plan ? plan->limb_elems : 0;
 }

@@ -462,9 +462,21 @@ class Contraction:
 
     def tune(self, reps=3, force=False, warmup=True):
         """Autotunes the tile configuration on the current work list (built
-        by worklist_step) and keeps it; FP64 and FP32 only."""
+        by worklist_step, with the limbs split for a split precision) and
+        keeps it; FP64, FP32 and BF16-limb precisions."""
+        if self.nslices:
+            cfg = _lib.GemmConfig()
+            ms = ctypes.c_float()
+            flags = (_lib.TUNE_FORCE if force else 0) | (0 if warmup else _lib.TUNE_NO_WARMUP)
+            call("tamx_autotune_grouped_gemm_split", self.ctx.ptr, self.wl.ptr, self.a_split.ptr,
+                 self.a_split.limbs.data_ptr(), self.b_split.ptr, self.b_split.limbs.data_ptr(),
+                 self.C.data_ptr(), int(reps), flags, ctypes.byref(cfg), ctypes.byref(ms),
+                 self.ctx.stream())
+            tile = (cfg.bm, cfg.bn, cfg.bk, cfg.wgs_per_cu)
+            self.set_tile(tile)
+            return tile, ms.value
         if self.precision not in ("f64", "f32"):
-            raise ValueError("autotuning covers the f64 and f32 grouped GEMMs")
+            raise ValueError("autotuning covers the f64, f32 and BF16-limb grouped GEMMs")
         tile, ms = autotune(self.ctx, self.wl, self.C.dtype, self.op_a, self.op_b, self.A.pool,
                             self.B.pool, self.C, reps=reps, force=force, warmup=warmup)
         self.set_tile(tile)
