@@ -383,10 +383,13 @@ def bench_ladder(args):
                      "avg_launch_ms": round(gemm_ms, 4)})
     # bandwidth-bound kernels: the V permute (each stored element read and
     # written once), the norm passes over T and V, the limb split
-    bw = {"permute": (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3)),
-          "tile_norms": (stored_bytes(T) + stored_bytes(V),
+    # (a split precision reads V through an index map while splitting it:
+    # no permute pass, no permuted copy)
+    bw = {"tile_norms": (stored_bytes(T) + stored_bytes(V),
                          time_kernel(lambda: [E.tile_norms(ctx, X, out=X.shape_norms)
                                               for X in (T, V)], reps=3))}
+    if hasattr(c.B, "_perm_plan"):
+        bw["permute"] = (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3))
     if c.nslices:
         # FP64 read, S BF16 limbs written, for both operands
         nel = (stored_bytes(T) + stored_bytes(c.B)) // 8

@@ -337,6 +337,21 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
                            int64_t ntiles, const int64_t* host_offsets,
                            const int32_t* host_rows, const int32_t* host_cols,
                            tamx_split_plan** plan);
+/* A plan whose limb layout is read from the operand's stored tiles through
+ * a general index map, so an operand that needs a general permutation
+ * (permopt.h:254-381; permuted per tile on read, array_eval.h:330-365) is
+ * split without a permuted copy: limb (row, k) of tile t reads source
+ * element host_offsets[t] + rowoff(row) + koff(k), where rowoff is the
+ * mixed-radix sum over nrow_modes modes (extents host_row_ext[t*nrow_modes
+ * + d], source strides host_row_str[...], last mode fastest) and koff the
+ * same over nk_modes modes (1 to 4 each).  The limb layout and GEMM use are
+ * those of tamx_split_plan_create. */
+int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntiles,
+                                   const int64_t* host_offsets, int nrow_modes,
+                                   int nk_modes, const int32_t* host_row_ext,
+                                   const int64_t* host_row_str,
+                                   const int32_t* host_k_ext,
+                                   const int64_t* host_k_str, tamx_split_plan** plan);
 int tamx_split_plan_destroy(tamx_split_plan* plan);
 /* Limb count of a plan (0 for a null plan). */
 int tamx_split_plan_nslices(const tamx_split_plan* plan);

@@ -227,6 +227,46 @@ def test_split_error_drops_nine_bits_per_limb(ctx):
         assert 2.0 ** 7 <= ratio <= 2.0 ** 11, (S, errs)
 
 
+@pytest.mark.parametrize("nslices", [3, 4])
+def test_split_fused_permute_matches_permuted_copy(ctx, nslices):
+    # V stored (a,c,b,d) is split straight from its stored layout through an
+    # index map; the limbs and the result equal those from a permuted copy
+    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=4)
+    cf = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d",
+                       precision=f"bf16x{nslices}")
+    cp = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d",
+                       precision=f"bf16x{nslices}", fuse_permute=False)
+    assert hasattr(cf.B, "_view_src") and hasattr(cp.B, "_perm_plan")
+    cf.run()
+    cp.run()
+    torch.cuda.synchronize()
+    assert torch.equal(cf.b_split.limbs, cp.b_split.limbs)
+    assert torch.equal(cf.a_split.limbs, cp.a_split.limbs)
+    np.testing.assert_array_equal(cf.result().to_dense(), cp.result().to_dense())
+
+
+@pytest.mark.parametrize("annot", ["c,a,d,b", "d,c,b,a", "b,d,a,c"])
+def test_split_fused_permute_other_orders(ctx, annot):
+    # other general permutations of the right operand (the stride-1 source
+    # mode lands in the limb rows or in k), integer data: exact
+    rng = np.random.default_rng(11)
+    tv = TiledRange1([0, 5, 12, 20])
+    to = TiledRange1([0, 3, 6])
+    T = DistArray.allocate(TiledRange([tv, tv, to, to]), rng.random(36) < 0.9)
+    T.pool.copy_(torch.as_tensor(rng.integers(-8, 9, T.pool.numel()), dtype=torch.float64))
+    V = DistArray.allocate(TiledRange([tv, tv, tv, tv]), rng.random(81) < 0.9)
+    V.pool.copy_(torch.as_tensor(rng.integers(-8, 9, V.pool.numel()), dtype=torch.float64))
+    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, annot, precision="bf16x3")
+    c.run()
+    R = c.result().to_dense()
+    ref = np.einsum("cdij," + annot.replace(",", "") + "->abij", T.to_dense(), V.to_dense())
+    np.testing.assert_array_equal(R, ref)
+    # the same operand on the left
+    c = E.Contraction(ctx, "i,j,a,b", V, annot, T, "c,d,i,j", precision="bf16x3")
+    c.run()
+    np.testing.assert_array_equal(c.result().to_dense(), ref.transpose(2, 3, 0, 1))
+
+
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
 def test_f32x2_integer_exact(ctx, op_a, op_b):
     # small integers split exactly into hi (lo = 0) and their products and

@@ -127,6 +127,7 @@ SIGNATURES = {
     "tamx_grouped_gemm_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp],
     "tamx_split_plan_create": [_vp, _i, _i, _i, _i64, _vp, _vp, _vp, _vp],
     "tamx_split_plan_destroy": [_vp],
+    "tamx_split_plan_create_general": [_vp, _i, _i64, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "tamx_split_plan_limb_elems": [_vp],
     "tamx_split_plan_sigmas": [_vp, _vp],
     "tamx_split_f64_premax": [_vp, _vp, _vp, _vp, _vp, _vp],

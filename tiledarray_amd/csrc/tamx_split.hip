@@ -66,10 +66,60 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
 // The 8 values of thread t's (row, 8 k) in block (r0, k0) of tile t_, zero
 // outside the tile.  A transposed tile (stored k x rows) is staged through
 // `stg` so the global reads run along the stored rows.
+__device__ inline int64_t mixed_radix_offset(int x, int n, const int32_t* ext,
+                                             const int64_t* str) {
+  int64_t off = 0;
+  for (int d = n - 1; d >= 0; --d) {
+    const int e = ext[d];
+    const int q = x / e;
+    off += (int64_t)(x - q * e) * str[d];
+    x = q;
+  }
+  return off;
+}
+
 __device__ inline void load_block8(const double* __restrict__ pool, const SplitTile& t, int r0,
                                    int k0, double (*stg)[kSplitBR + 1], double (&v)[8]) {
   const int tid = threadIdx.x;
   const int r = tid >> 2, q = tid & 3;
+  if (t.gen_nr > 0) {
+    // general layout, staged through LDS like a transposed tile: the lanes
+    // run along the rows when the last row mode is the source's stride-1
+    // mode, else along k, so consecutive lanes read consecutive addresses
+    const bool rows_fast = t.gr_str[t.gen_nr - 1] == 1;
+    if (rows_fast) {
+      const int rr = tid & 63, gr = r0 + rr;
+      const bool row_ok = gr < t.out_rows;
+      const int64_t base =
+          t.src_off + (row_ok ? mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str) : 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = (tid >> 6) + 4 * j, gk = k0 + kk;
+        double x = 0.0;
+        if (row_ok && gk < t.out_k)
+          x = pool[base + mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str)];
+        stg[kk][rr] = x;
+      }
+    } else {
+      const int kk = tid & 31, gk = k0 + kk;
+      const bool k_ok = gk < t.out_k;
+      const int64_t base =
+          t.src_off + (k_ok ? mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str) : 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int rr = (tid >> 5) + 8 * j, gr = r0 + rr;
+        double x = 0.0;
+        if (k_ok && gr < t.out_rows)
+          x = pool[base + mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str)];
+        stg[kk][rr] = x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
+    __syncthreads();
+    return;
+  }
   if (t.transpose) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -828,6 +878,86 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
     TAMX_REQUIRE(st.rows >= 0 && st.cols >= 0, "negative tile extent");
     st.out_rows = transpose ? st.cols : st.rows;
     st.out_k = transpose ? st.rows : st.cols;
+    st.rows_pad = (st.out_rows + 15) / 16 * 16;
+    st.k_pad = std::max(32, (st.out_k + 31) / 32 * 32);
+    st.limb_off = off;
+    off += (int64_t)nslices * st.rows_pad * st.k_pad;
+    max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
+    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * (st.k_pad / kSplitBK);
+  }
+  prefix[ntiles] = blocks;
+  auto* plan = new (std::nothrow) tamx_split_plan();
+  if (!plan) return TAMX_ERR_OOM;
+  plan->ctx = ctx;
+  plan->ntiles = ntiles;
+  plan->nslices = nslices;
+  plan->limb_elems = off + (int64_t)kLimbSlackRows * max_kpad * nslices;
+  plan->nblocks = blocks;
+  hipError_t e = hipMalloc(&plan->tiles, std::max<size_t>(1, tiles.size()) * sizeof(SplitTile));
+  if (e == hipSuccess) e = hipMalloc(&plan->block_prefix, prefix.size() * sizeof(int64_t));
+  if (e == hipSuccess)
+    e = hipMalloc(&plan->maxbits, std::max<size_t>(1, tiles.size()) * sizeof(unsigned long long));
+  if (e == hipSuccess && !tiles.empty())
+    e = hipMemcpy(plan->tiles, tiles.data(), tiles.size() * sizeof(SplitTile), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->block_prefix, prefix.data(), prefix.size() * sizeof(int64_t),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tamx::set_error(std::string("split plan: ") + hipGetErrorString(e));
+    if (plan->tiles) (void)hipFree(plan->tiles);
+    if (plan->block_prefix) (void)hipFree(plan->block_prefix);
+    if (plan->maxbits) (void)hipFree(plan->maxbits);
+    delete plan;
+    return e == hipErrorOutOfMemory ? TAMX_ERR_OOM : TAMX_ERR_DEVICE;
+  }
+  *out = plan;
+  return TAMX_OK;
+}
+
+int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntiles,
+                                   const int64_t* host_offsets, int nrow_modes, int nk_modes,
+                                   const int32_t* host_row_ext, const int64_t* host_row_str,
+                                   const int32_t* host_k_ext, const int64_t* host_k_str,
+                                   tamx_split_plan** out) {
+  TAMX_REQUIRE(out != nullptr, "null output pointer");
+  *out = nullptr;
+  TAMX_REQUIRE(nslices >= 2 && nslices <= 6, "nslices must be in [2, 6]");
+  TAMX_REQUIRE(ntiles >= 0, "negative tile count");
+  TAMX_REQUIRE(nrow_modes >= 1 && nrow_modes <= 4 && nk_modes >= 1 && nk_modes <= 4,
+               "row and k modes must number 1 to 4 each");
+  TAMX_REQUIRE(ntiles == 0 || (host_offsets && host_row_ext && host_row_str && host_k_ext &&
+                               host_k_str),
+               "null tile table");
+  if (int rc = tamx::activate(ctx)) return rc;
+  std::vector<SplitTile> tiles(ntiles);
+  std::vector<int64_t> prefix(ntiles + 1, 0);
+  int64_t off = 0, blocks = 0, max_kpad = 32;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    SplitTile& st = tiles[t];
+    st = SplitTile{};
+    st.src_off = host_offsets[t];
+    prefix[t] = blocks;
+    if (st.src_off < 0) continue;  // absent tile: no limbs
+    int64_t nr = 1, nk = 1;
+    st.gen_nr = nrow_modes;
+    st.gen_nk = nk_modes;
+    for (int d = 0; d < nrow_modes; ++d) {
+      st.gr_ext[d] = host_row_ext[t * nrow_modes + d];
+      st.gr_str[d] = host_row_str[t * nrow_modes + d];
+      TAMX_REQUIRE(st.gr_ext[d] >= 0 && st.gr_str[d] >= 0, "negative extent or stride");
+      nr *= st.gr_ext[d];
+    }
+    for (int d = 0; d < nk_modes; ++d) {
+      st.gk_ext[d] = host_k_ext[t * nk_modes + d];
+      st.gk_str[d] = host_k_str[t * nk_modes + d];
+      TAMX_REQUIRE(st.gk_ext[d] >= 0 && st.gk_str[d] >= 0, "negative extent or stride");
+      nk *= st.gk_ext[d];
+    }
+    TAMX_REQUIRE(nr < (int64_t(1) << 31) && nk < (int64_t(1) << 31), "tile too large");
+    st.out_rows = (int32_t)nr;
+    st.out_k = (int32_t)nk;
+    st.rows = st.out_rows;
+    st.cols = st.out_k;
     st.rows_pad = (st.out_rows + 15) / 16 * 16;
     st.k_pad = std::max(32, (st.out_k + 31) / 32 * 32);
     st.limb_off = off;

@@ -22,6 +22,14 @@ struct SplitTile {
   int32_t k_pad;       // out_k rounded up to 32 (row stride of a plane)
   int32_t transpose;   // limb (row, k) reads stored (k, row)
   int32_t sigma;       // scale exponent, written by the max pass
+  // general layout (gen_nr > 0): limb (row, k) reads source element
+  // src_off + rowoff(row) + koff(k), each a mixed-radix sum over up to four
+  // source modes (last fastest) with the modes' source strides; this is how
+  // an operand that needs a general permutation is split straight from its
+  // stored layout, without a permuted copy
+  int32_t gen_nr, gen_nk;
+  int32_t gr_ext[4], gk_ext[4];
+  int64_t gr_str[4], gk_str[4];
 };
 
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));

@@ -284,6 +284,27 @@ def plan_contraction(left, right, target, prefer_to_permute_left=True):
     return p
 
 
+def permuted_view(arr, perm):
+    """The TO-permuted array's structure (tiled range, tile offsets, shape)
+    over the *unpermuted* source pool, for consumers that read the source
+    through an index map (the limb split): no permuted copy is made.  The
+    view records its source (`_view_src`, `_view_perm`); its pool must not
+    be read as if permuted."""
+    perm = [int(x) for x in perm]
+    new_tr = arr.trange.permuted(perm)
+    grid = np.arange(arr.trange.ntiles).reshape(arr.trange.tiles_shape)
+    inv = np.argsort(perm)
+    src_of_new = np.transpose(grid, inv).reshape(-1)  # source ordinal of each permuted tile
+    new_offs = arr.offsets[src_of_new].copy()
+    out = DistArray(new_tr, arr.pool, new_offs)
+    if arr.shape_norms is not None:
+        out.shape_norms = arr.shape_norms.view(arr.trange.tiles_shape).permute(
+            *[int(i) for i in inv]).contiguous().view(-1)
+        out.shape_threshold = arr.shape_threshold
+    out._view_src, out._view_perm, out._view_src_of = arr, perm, src_of_new
+    return out
+
+
 def permute_array(ctx, arr, perm):
     """TO-permuted copy of `arr` (tile data, tiled range and shape)."""
     perm = [int(x) for x in perm]
@@ -322,7 +343,8 @@ class Contraction:
     """
 
     def __init__(self, ctx, target, A, left, B, right, alpha=1.0, threshold=FLOAT_EPS,
-                 pair_threshold=0.0, precision="f64", tile=None, permute_operands=True):
+                 pair_threshold=0.0, precision="f64", tile=None, permute_operands=True,
+                 fuse_permute=True):
         self.ctx = ctx
         self.alpha = float(alpha)
         self.precision = precision
@@ -330,12 +352,18 @@ class Contraction:
         self.plan = plan = plan_contraction(left, right, target)
         self.threshold = float(np.float32(threshold))
         dev = A.pool.device
-        # operands in GEMM layout
+        # operands in GEMM layout.  A split precision reads a `general`
+        # operand through an index map while splitting it (no permuted
+        # copy) unless fuse_permute is off.
+        fuse = bool(fuse_permute) and precision.startswith("bf16x")
         self.A_src, self.B_src = A, B
-        self.A = permute_array(ctx, A, plan.left_perm[:plan.left_rank]) \
-            if plan.left_permtype == 3 else A
-        self.B = permute_array(ctx, B, plan.right_perm[:plan.right_rank]) \
-            if plan.right_permtype == 3 else B
+
+        def gemm_layout(X, permtype, perm):
+            if permtype != 3:
+                return X
+            return permuted_view(X, perm) if fuse else permute_array(ctx, X, perm)
+        self.A = gemm_layout(A, plan.left_permtype, plan.left_perm[:plan.left_rank])
+        self.B = gemm_layout(B, plan.right_permtype, plan.right_perm[:plan.right_rank])
         self.permute_operands = permute_operands
         Ag, Bg = self.A, self.B
         l_inner = [bool(plan.left_gemm_is_inner[d]) for d in range(plan.left_rank)]
@@ -386,6 +414,8 @@ class Contraction:
     def _split_plan(self, X, is_right, op):
         """Split plan over the operand's fused tile grid (ordinals as in the
         work list's pairs) and its BF16 limb pool."""
+        if hasattr(X, "_view_src"):
+            return self._split_plan_general(X, is_right)
         Mt, Nt, Kt = self.Mt, self.Nt, self.Kt
         if not is_right:
             outer, rows_first = self.mext_h, not op  # NoTrans: (m, k)
@@ -409,6 +439,44 @@ class Contraction:
         n = int(lib().tamx_split_plan_limb_elems(p))
         limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
         return _SplitOperand(p, limbs, offs.size)
+
+    def _split_plan_general(self, X, is_right):
+        """Split plan of a permuted view: limb rows are the operand's outer
+        (non-contracted) modes, limb k its contracted modes, both in GEMM
+        order, each mapped to its stride in the stored source tile."""
+        plan = self.plan
+        rank = plan.right_rank if is_right else plan.left_rank
+        inner = [bool((plan.right_gemm_is_inner if is_right else plan.left_gemm_is_inner)[d])
+                 for d in range(rank)]
+        src, perm, src_of = X._view_src, X._view_perm, X._view_src_of
+        inv = np.argsort(perm)  # permuted mode d' is source mode inv[d']
+        sext = src.trange.tile_extents_table().astype(np.int64)  # by source ordinal
+        row_modes = [d for d in range(rank) if not inner[d]]
+        k_modes = [d for d in range(rank) if inner[d]]
+        if not (1 <= len(row_modes) <= 4 and 1 <= len(k_modes) <= 4):
+            raise ValueError("fused permute-on-split covers 1 to 4 outer and inner modes")
+        nt = X.trange.ntiles
+        rext = np.zeros((nt, len(row_modes)), np.int32)
+        rstr = np.zeros((nt, len(row_modes)), np.int64)
+        kext = np.zeros((nt, len(k_modes)), np.int32)
+        kstr = np.zeros((nt, len(k_modes)), np.int64)
+        for t in range(nt):
+            e = sext[src_of[t]]
+            strides = np.ones(rank, np.int64)  # row-major strides of the source tile
+            for d in range(rank - 2, -1, -1):
+                strides[d] = strides[d + 1] * e[d + 1]
+            for w, dp in enumerate(row_modes):
+                rext[t, w], rstr[t, w] = e[inv[dp]], strides[inv[dp]]
+            for w, dp in enumerate(k_modes):
+                kext[t, w], kstr[t, w] = e[inv[dp]], strides[inv[dp]]
+        offs = np.ascontiguousarray(X.offsets, np.int64)
+        p = ctypes.c_void_p()
+        call("tamx_split_plan_create_general", self.ctx.ptr, self.nslices, nt, offs.ctypes.data,
+             len(row_modes), len(k_modes), rext.ctypes.data, rstr.ctypes.data, kext.ctypes.data,
+             kstr.ctypes.data, ctypes.byref(p))
+        n = int(lib().tamx_split_plan_limb_elems(p))
+        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
+        return _SplitOperand(p, limbs, nt)
 
     def split_step(self, maxabs=None):
         """Split both operands into BF16 limbs (part of the split path).
@@ -486,7 +554,7 @@ class Contraction:
     def permute_step(self):
         """Re-permutes `general` operands from their source arrays."""
         for src, dst in ((self.A_src, self.A), (self.B_src, self.B)):
-            if dst is not src:
+            if dst is not src and hasattr(dst, "_perm_plan"):
                 dst._perm_plan.run(src.pool, dst.pool)
 
     def shape_step(self):
