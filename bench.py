@@ -59,6 +59,10 @@ def parse():
                    help="split precisions measured beside the FP64 run")
     p.add_argument("--split-tile", default=None,
                    help="macro tile BMxBN of the split GEMM (64x64, 128x64 or 64x128)")
+    p.add_argument("--v-permute", default="auto", choices=["auto", "copy", "map"],
+                   help="ladder: permute V into a copy each step, or read it through the "
+                        "permutation's index map (auto: the map for split precisions, the "
+                        "copy for FP64)")
     return p.parse_args()
 
 
@@ -313,9 +317,11 @@ def base_line(args, value, ms, dtype, workload, config, roofline):
 
 def bench_ladder(args):
     """R(a,b,i,j) = T(c,d,i,j) * V(a,c,b,d) with V stored to force a tile
-    permutation; one step = tile norms and shapes of T and V, the V permute,
-    (the limb split), the result estimate, the work list and the grouped
-    GEMM with the result transpose fused into its store."""
+    permutation; one step = tile norms and shapes of T and V, (the limb
+    split), the result estimate, the work list and the grouped GEMM with the
+    result transpose fused into its store.  V is permuted into a copy in the
+    step (as the reference permutes on read) or read through the
+    permutation's index map by the limb split or the GEMM (--v-permute)."""
     import numpy as np
     import torch
 
@@ -330,7 +336,8 @@ def bench_ladder(args):
     E.make_sparse(ctx, V, thr)
     sizes = {id(X): E._sizes_vector(X.trange, X.pool.device) for X in (T, V)}
     c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", threshold=thr,
-                      precision=args.precision, tile=fixed_tile(args))
+                      precision=args.precision, tile=fixed_tile(args),
+                      fuse_permute={"auto": None, "copy": False, "map": True}[args.v_permute])
     # the permuted V's shape follows the source's; refresh it each step
     perm_norms = c.B.shape_norms
 
@@ -376,7 +383,8 @@ def bench_ladder(args):
                                                  else [64, 64, 32]),
                      "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
-                     "precision": args.precision, "seed": info["seed"]},
+                     "precision": args.precision, "seed": info["seed"],
+                     "v_permute": "copy" if hasattr(c.B, "_perm_plan") else "index map"},
                     {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,
@@ -390,6 +398,12 @@ def bench_ladder(args):
                                               for X in (T, V)], reps=3))}
     if hasattr(c.B, "_perm_plan"):
         bw["permute"] = (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3))
+    elif not c.nslices:
+        # the permute kernel on V outside the step (into a scratch copy)
+        Y = E.permute_array(ctx, V, list(c.plan.right_perm[:4]))
+        bw["permute_standalone"] = (2 * stored_bytes(V),
+                                    time_kernel(lambda: Y._perm_plan.run(V.pool, Y.pool), reps=3))
+        del Y
     if c.nslices:
         # FP64 read, S BF16 limbs written, for both operands
         nel = (stored_bytes(T) + stored_bytes(c.B)) // 8

@@ -316,6 +316,41 @@ int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, float alpha, const float* A,
                           const float* B, float* C, void* stream);
 
+/* Operands read through an index map.  An operand that needs a general
+ * permutation into GEMM layout (permopt.h:254-381) is permuted per tile on
+ * read in the reference (array_eval.h:330-365 -> Tensor(other, perm),
+ * tensor.h:644-692) before gemm; here the GEMM reads its stored tiles
+ * directly and no permuted copy exists.  A gather plan lists, for every
+ * tile ordinal the work list's pairs carry (a_tile / b_tile), the extents
+ * and stored strides of the tile's outer modes (rows of op(X) in GEMM
+ * order, host_row_ext/str[t * nrow_modes + d]) and of its contracted modes
+ * (host_k_ext/str[t * nk_modes + d]), 1 to 4 of each, the last fastest:
+ * element (row, k) of tile t lies at the pair's tile offset + rowoff(row)
+ * + koff(k).  The products of the extents must equal the tile's m (or n)
+ * and k in the work list. */
+typedef struct tamx_gather_plan tamx_gather_plan;
+int tamx_gather_plan_create(tamx_context* ctx, int64_t ntiles, int nrow_modes,
+                            int nk_modes, const int32_t* host_row_ext,
+                            const int64_t* host_row_str, const int32_t* host_k_ext,
+                            const int64_t* host_k_str, tamx_gather_plan** plan);
+int tamx_gather_plan_destroy(tamx_gather_plan* plan);
+/* The grouped GEMM with either operand (or both) gathered; a null plan
+ * reads that operand as tamx_grouped_gemm_*_config does.  For a gathered
+ * operand the pair's leading dimension is unused and its op only picks the
+ * index the loads run along (A: NO_TRANS k, TRANS m; B: NO_TRANS n, TRANS
+ * k): pass the one whose last mode is the stored stride-1 mode.  Results
+ * equal those on a permuted copy bit for bit. */
+int tamx_grouped_gemm_gather_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                 int op_b, double alpha, const double* A,
+                                 const tamx_gather_plan* a_gather, const double* B,
+                                 const tamx_gather_plan* b_gather, double* C,
+                                 const tamx_gemm_config* cfg, void* stream);
+int tamx_grouped_gemm_gather_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                 int op_b, float alpha, const float* A,
+                                 const tamx_gather_plan* a_gather, const float* B,
+                                 const tamx_gather_plan* b_gather, float* C,
+                                 const tamx_gemm_config* cfg, void* stream);
+
 /* ------------------------------------------------ split precision --- */
 
 /* Splitting of FP64 operand tiles into nslices BF16 limbs with one
@@ -418,6 +453,14 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl,
                                const void* A, const void* B, const void* C,
                                int reps, int flags, tamx_gemm_config* best,
                                float* best_ms, void* stream);
+/* The same for tamx_grouped_gemm_gather_f64 / _f32 (null plans allowed). */
+int tamx_autotune_grouped_gemm_gather(tamx_context* ctx, tamx_worklist* wl,
+                                      int precision, int op_a, int op_b,
+                                      const void* A, const tamx_gather_plan* a_gather,
+                                      const void* B, const tamx_gather_plan* b_gather,
+                                      const void* C, int reps, int flags,
+                                      tamx_gemm_config* best, float* best_ms,
+                                      void* stream);
 /* The same for the split GEMM (tamx_grouped_gemm_split_config): times its
  * macro tiles (64x64; up to four limbs also 128x64 and 64x128) and returns
  * the fastest; cached per device, limb count and shape class. */

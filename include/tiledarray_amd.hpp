@@ -254,6 +254,9 @@ struct ContractOptions {
   Precision precision = Precision::FP64;
   int nslices = 5;                        // BF16 limbs for BF16_LIMBS
   bool autotune = false;                  // pick the tile shape by timing (FP64/FP32)
+  // BF16_LIMBS: a `general` operand is split straight from its stored tiles
+  // through the permutation's index map (no permuted copy) unless set
+  bool permute_copy = false;
 };
 
 template <typename T>
@@ -269,7 +272,8 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
               const std::string& la, const DistArray<T>& B, const std::string& lb, double factor,
               const ContractOptions& opt);
 template <typename T>
-DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& perm);
+DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& perm,
+                           bool copy_data = true, std::vector<size_t>* src_of = nullptr);
 }  // namespace detail
 
 // A block-sparse tiled tensor resident in HBM: the non-zero tiles back to
@@ -376,7 +380,8 @@ class DistArray {
                                const std::string&, const DistArray<U>&, const std::string&,
                                double, const ContractOptions&);
   template <typename U>
-  friend DistArray<U> detail::permute_array(World&, const DistArray<U>&, const Permutation&);
+  friend DistArray<U> detail::permute_array(World&, const DistArray<U>&, const Permutation&,
+                                            bool, std::vector<size_t>*);
 
   World* world_ = nullptr;
   TiledRange trange_;
@@ -456,15 +461,20 @@ inline std::vector<float> k_sizes(const std::vector<const TiledRange1*>& ranges,
   return out;
 }
 
-// TO-permuted copy: tile data through tamx_permute, shape norms reordered
+// TO-permuted copy: tile data through tamx_permute, shape norms reordered.
+// Without copy_data only the structure is permuted (tiled range, tile
+// offsets into a's pool, shape) and the result has no pool of its own;
+// src_of receives the source ordinal of every permuted tile.
 template <typename T>
-DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& perm) {
+DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& perm,
+                           bool copy_data, std::vector<size_t>* src_of) {
   const TiledRange& tr = a.trange();
   DistArray<T> out;
   out.world_ = &w;
   out.trange_ = tr.permute(perm.data());
   const size_t nt = tr.ntiles(), rank = tr.rank();
   out.offsets_.assign(nt, -1);
+  if (src_of) src_of->assign(nt, 0);
   std::vector<int64_t> ext, src, dst;
   for (size_t t = 0; t < nt; ++t) {
     auto idx = tr.tile_index(t);
@@ -473,6 +483,7 @@ DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& p
     size_t o = 0;
     for (size_t d = 0; d < rank; ++d) o = o * out.trange_.dim(d).ntiles() + pidx[d];
     out.offsets_[o] = a.offsets_[t];  // a tile keeps its place in the pool
+    if (src_of) (*src_of)[o] = t;
     if (a.offsets_[t] >= 0) {
       auto e = tr.tile_extents(t);
       ext.insert(ext.end(), e.begin(), e.end());
@@ -480,16 +491,16 @@ DistArray<T> permute_array(World& w, const DistArray<T>& a, const Permutation& p
       dst.push_back(a.offsets_[t]);
     }
   }
-  out.pool_ = DeviceBuffer<T>(std::max<int64_t>(1, a.pool_elems_));
   out.pool_elems_ = a.pool_elems_;
   out.d_offsets_ = to_device(out.offsets_);
+  if (copy_data) out.pool_ = DeviceBuffer<T>(std::max<int64_t>(1, a.pool_elems_));
   tamx_permute_plan* plan = nullptr;
   check(tamx_permute_plan_create(w.ctx(), (int)rank, perm.data().data(), (int64_t)src.size(),
                                  ext.data(), src.data(), dst.data(), &plan),
         "tamx_permute_plan_create");
   std::unique_ptr<tamx_permute_plan, int (*)(tamx_permute_plan*)> guard(plan,
                                                                        tamx_permute_plan_destroy);
-  if (!src.empty()) {
+  if (copy_data && !src.empty()) {
     if (std::is_same<T, double>::value)
       check(tamx_permute_f64(w.ctx(), plan, reinterpret_cast<const double*>(a.pool()),
                              reinterpret_cast<double*>(out.pool()), w.stream()),
@@ -533,18 +544,25 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
   if (A0.is_dense() != B0.is_dense())
     throw Exception(TAMX_ERR_BAD_ARG, "operands must both be dense or both be sparse");
   auto perm_of = [](const int32_t* p, int n) { return Permutation(std::vector<int>(p, p + n)); };
-  // operands whose layout is `general` are permuted (permute-on-read)
+  // operands whose layout is `general` are permuted (permute-on-read), or,
+  // for the limb split, only their structure is, and the split reads the
+  // stored tiles through the permutation's index map
+  const bool map = opt.precision == Precision::BF16_LIMBS && !opt.permute_copy;
   DistArray<T> Ap, Bp;
   const DistArray<T>* A = &A0;
   const DistArray<T>* B = &B0;
+  std::vector<size_t> a_src, b_src;
   if (plan.left_permtype == 3) {
-    Ap = permute_array(w, A0, perm_of(plan.left_perm, plan.left_rank));
+    Ap = permute_array(w, A0, perm_of(plan.left_perm, plan.left_rank), !map, &a_src);
     A = &Ap;
   }
   if (plan.right_permtype == 3) {
-    Bp = permute_array(w, B0, perm_of(plan.right_perm, plan.right_rank));
+    Bp = permute_array(w, B0, perm_of(plan.right_perm, plan.right_rank), !map, &b_src);
     B = &Bp;
   }
+  // element data of the operands in the layout the GEMM or split reads
+  const T* a_data = (map && A != &A0) ? A0.pool() : A->pool();
+  const T* b_data = (map && B != &B0) ? B0.pool() : B->pool();
   // fused structure (GemmHelper, math/gemm_helper.h:41-278)
   std::vector<const TiledRange1*> lo, li, ro, ri;
   for (int d = 0; d < plan.left_rank; ++d)
@@ -617,19 +635,19 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
     check(tamx_gemm_config_get(0, &cfg), "tamx_gemm_config_get");
     if (opt.autotune)
       check(tamx_autotune_grouped_gemm(w.ctx(), wl, f64 ? TAMX_PREC_F64 : TAMX_PREC_F32,
-                                       plan.op_a, plan.op_b, A->pool(), B->pool(), C.get(),
+                                       plan.op_a, plan.op_b, a_data, b_data, C.get(),
                                        3, 0, &cfg, nullptr, w.stream()),
             "tamx_autotune_grouped_gemm");
     if (f64)
       check(tamx_grouped_gemm_f64_config(w.ctx(), wl, plan.op_a, plan.op_b, factor,
-                                         reinterpret_cast<const double*>(A->pool()),
-                                         reinterpret_cast<const double*>(B->pool()),
+                                         reinterpret_cast<const double*>(a_data),
+                                         reinterpret_cast<const double*>(b_data),
                                          reinterpret_cast<double*>(C.get()), &cfg, w.stream()),
             "tamx_grouped_gemm_f64_config");
     else
       check(tamx_grouped_gemm_f32_config(w.ctx(), wl, plan.op_a, plan.op_b, (float)factor,
-                                         reinterpret_cast<const float*>(A->pool()),
-                                         reinterpret_cast<const float*>(B->pool()),
+                                         reinterpret_cast<const float*>(a_data),
+                                         reinterpret_cast<const float*>(b_data),
                                          reinterpret_cast<float*>(C.get()), &cfg, w.stream()),
             "tamx_grouped_gemm_f32_config");
   } else {
@@ -646,24 +664,68 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
         cols.push_back((int32_t)(rows_first ? kext[b] : outer[b]));
       }
     };
+    // the index map of a permuted operand: per tile of the permuted grid,
+    // extents and stored strides of its outer modes (limb rows) and its
+    // contracted modes (limb k), in GEMM order
+    auto general_plan = [&](const DistArray<T>& src, const DistArray<T>& view,
+                            const std::vector<size_t>& src_of, const int32_t* perm,
+                            const int32_t* is_inner, int rank) {
+      std::vector<int> inv(rank);
+      for (int d = 0; d < rank; ++d) inv[perm[d]] = d;
+      std::vector<int> rows_m, k_m;
+      for (int d = 0; d < rank; ++d) (is_inner[d] ? k_m : rows_m).push_back(d);
+      if (rows_m.empty() || k_m.empty() || rows_m.size() > 4 || k_m.size() > 4)
+        throw Exception(TAMX_ERR_UNSUPPORTED, "index maps cover 1 to 4 outer and inner modes");
+      const size_t nt = view.trange().ntiles();
+      std::vector<int32_t> rext, kext;
+      std::vector<int64_t> rstr, kstr;
+      for (size_t t = 0; t < nt; ++t) {
+        const auto e = src.trange().tile_extents(src_of[t]);
+        std::vector<int64_t> st(rank, 1);
+        for (int d = rank - 2; d >= 0; --d) st[d] = st[d + 1] * (int64_t)e[d + 1];
+        for (int dp : rows_m) {
+          rext.push_back((int32_t)e[inv[dp]]);
+          rstr.push_back(st[inv[dp]]);
+        }
+        for (int dp : k_m) {
+          kext.push_back((int32_t)e[inv[dp]]);
+          kstr.push_back(st[inv[dp]]);
+        }
+      }
+      tamx_split_plan* sp = nullptr;
+      check(tamx_split_plan_create_general(w.ctx(), opt.nslices, (int64_t)nt,
+                                           view.offsets().data(), (int)rows_m.size(),
+                                           (int)k_m.size(), rext.data(), rstr.data(),
+                                           kext.data(), kstr.data(), &sp),
+            "tamx_split_plan_create_general");
+      return sp;
+    };
     std::vector<int32_t> ar, ac, br, bc;
     table(false, plan.op_a, ar, ac);
     table(true, plan.op_b, br, bc);
     tamx_split_plan *pa = nullptr, *pb = nullptr;
-    check(tamx_split_plan_create(w.ctx(), opt.nslices, 0, plan.op_a, (int64_t)ar.size(),
-                                 A->offsets().data(), ar.data(), ac.data(), &pa),
-          "tamx_split_plan_create");
+    if (map && A != &A0)
+      pa = general_plan(A0, *A, a_src, plan.left_perm, plan.left_gemm_is_inner,
+                        plan.left_rank);
+    else
+      check(tamx_split_plan_create(w.ctx(), opt.nslices, 0, plan.op_a, (int64_t)ar.size(),
+                                   A->offsets().data(), ar.data(), ac.data(), &pa),
+            "tamx_split_plan_create");
     sa.reset(pa);
-    check(tamx_split_plan_create(w.ctx(), opt.nslices, 1, plan.op_b, (int64_t)br.size(),
-                                 B->offsets().data(), br.data(), bc.data(), &pb),
-          "tamx_split_plan_create");
+    if (map && B != &B0)
+      pb = general_plan(B0, *B, b_src, plan.right_perm, plan.right_gemm_is_inner,
+                        plan.right_rank);
+    else
+      check(tamx_split_plan_create(w.ctx(), opt.nslices, 1, plan.op_b, (int64_t)br.size(),
+                                   B->offsets().data(), br.data(), bc.data(), &pb),
+            "tamx_split_plan_create");
     sb.reset(pb);
     la_limbs = DeviceBuffer<uint16_t>(std::max<int64_t>(1, tamx_split_plan_limb_elems(pa)));
     lb_limbs = DeviceBuffer<uint16_t>(std::max<int64_t>(1, tamx_split_plan_limb_elems(pb)));
-    check(tamx_split_f64(w.ctx(), pa, reinterpret_cast<const double*>(A->pool()), la_limbs.get(),
+    check(tamx_split_f64(w.ctx(), pa, reinterpret_cast<const double*>(a_data), la_limbs.get(),
                          w.stream()),
           "tamx_split_f64");
-    check(tamx_split_f64(w.ctx(), pb, reinterpret_cast<const double*>(B->pool()), lb_limbs.get(),
+    check(tamx_split_f64(w.ctx(), pb, reinterpret_cast<const double*>(b_data), lb_limbs.get(),
                          w.stream()),
           "tamx_split_f64");
     check(tamx_grouped_gemm_split(w.ctx(), wl, factor, pa, la_limbs.get(), pb, lb_limbs.get(),

@@ -230,6 +230,19 @@ int main() {
     CHECK(err <= 1e-10, "relative error %g", err);
   });
 
+  run("split precision, general V read through the index map = permuted copy", [&] {
+    auto T = make_array<double>(world, TiledRange{tv, tv, to, to}, rng, false);
+    auto V = make_array<double>(world, TiledRange{tv, tv, tv, tv}, rng, false);
+    DistArray<double> Rm(world, TiledRange{tv, tv, to, to}), Rc(world, TiledRange{tv, tv, to, to});
+    ContractOptions opt;
+    opt.precision = Precision::BF16_LIMBS;
+    opt.nslices = 3;
+    Rm("a,b,i,j") = (T("c,d,i,j") * V("a,c,b,d")).with(opt);
+    opt.permute_copy = true;
+    Rc("a,b,i,j") = (T("c,d,i,j") * V("a,c,b,d")).with(opt);
+    CHECK(dense(Rm) == dense(Rc), "index-map split differs from the permuted-copy split");
+  });
+
   run("autotuned tile shape, block-sparse", [&] {
     TiledRange1 tb{0, 37, 90, 150, 151, 260, 333};
     auto A = make_array<double>(world, TiledRange{tb, tb}, rng, true, 0.7);

@@ -141,6 +141,12 @@ SIGNATURES = {
     "tamx_autotune_grouped_gemm_split": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
                                          _vp],
     "tamx_split_plan_nslices": [_vp],
+    "tamx_gather_plan_create": [_vp, _i64, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "tamx_gather_plan_destroy": [_vp],
+    "tamx_grouped_gemm_gather_f64": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_gather_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_autotune_grouped_gemm_gather": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                                          _vp, _vp, _vp],
     "tamx_graph_begin": [_vp, _vp],
     "tamx_graph_end": [_vp, _vp, _vp],
     "tamx_graph_launch": [_vp, _vp],

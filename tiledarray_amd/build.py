@@ -32,6 +32,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 # copies around every MFMA chain
 FILE_FLAGS = {"tamx_shape.hip": ["-ffp-contract=off"],
               "tamx_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "tamx_gemm_gather.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "tamx_gemm_gather_f32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
               "tamx_split.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
               "tamx_split_wide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 

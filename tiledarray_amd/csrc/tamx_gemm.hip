@@ -1,69 +1,9 @@
-// Persistent grouped GEMM over a device work list (FP64 and FP32 MFMA).
-//
-// One launch computes every result tile of a contraction.  A result tile of
-// m x n is cut into BM x BN macro tiles ("items"); a workgroup takes items
-// from a device queue, and for its item walks all contributing tile pairs in
-// order, K in chunks of BK, with A and B chunks staged through LDS (double
-// buffered, one barrier per chunk) and accumulated in MFMA registers.  The
-// result is written once: alpha*acc, plus the old C when the task asks for
-// beta = 1.  Output-stationary accumulation keeps every result bit-for-bit
-// reproducible (the pair order is the work-list order) and needs no atomics
-// on C.
-//
-// MFMA fragment layout of v_mfma_{f64,f32}_16x16x4: lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16].  The 16x16 result is held in
-// column l % 16; accumulator component v is row 4 * (l / 16) + v for f32 and
-// row (l / 16) + 4 * v for f64.
-#include <algorithm>
-#include <mutex>
-#include <unordered_map>
-
-#include "tamx_internal.hpp"
+// Plain entry points of the grouped GEMM and its item planner (the kernel
+// is in tamx_gemm.hpp).
+#include "tamx_gemm.hpp"
 #include "tamx_scan.hpp"
 
 namespace {
-
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  using acc_t = d4_t;
-  __device__ static inline acc_t op(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lk, int v) { return lk + 4 * v; }
-};
-template <>
-struct Mfma<float> {
-  using acc_t = f4_t;
-  __device__ static inline acc_t op(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lk, int v) { return 4 * lk + v; }
-};
-
-template <typename T>
-struct GemmParams {
-  const T* __restrict__ A;
-  const T* __restrict__ B;
-  T* __restrict__ C;
-  const tamx_gemm_task* __restrict__ tasks;
-  const tamx_gemm_pair* __restrict__ pairs;
-  const int64_t* __restrict__ counters;
-  const int32_t* __restrict__ prefix;
-  const int32_t* __restrict__ item_task;
-  int32_t* queue;
-  int64_t task_capacity;
-  int64_t item_capacity;
-  T alpha;
-};
-
-constexpr int kThreads = 256;
-constexpr int kRegions = 8;   // XCDs of an MI355X; the queue holds kRegions + 1 counters
-constexpr int kPairWin = 64;  // pair descriptors staged in LDS at a time
 
 // Item planner: prefix[t] = number of BM x BN macro tiles of tasks 0..t-1,
 // and the item -> task map when it fits in item_capacity entries.  Three
@@ -118,385 +58,6 @@ __global__ __launch_bounds__(kPlanThreads) void plan_items_fill_kernel(
   if (blockIdx.x == 0 && threadIdx.x == 0) prefix[n] = total;
 }
 
-// the address out-of-tile lanes load from
-__device__ const double kZeroF64 = 0.0;
-__device__ const float kZeroF32 = 0.0f;
-template <typename T>
-__device__ inline const T* zero_of();
-template <>
-__device__ inline const double* zero_of<double>() { return &kZeroF64; }
-template <>
-__device__ inline const float* zero_of<float>() { return &kZeroF32; }
-
-__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ inline int64_t uni64(int64_t x) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-template <typename T, int BM, int BN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p) {
-  using M = Mfma<T>;
-  using acc_t = typename M::acc_t;
-  constexpr int FM = BM / 32, FN = BN / 32;  // 16x16 blocks per wave (2x2 waves)
-  constexpr int A_LD = BM * BK / kThreads, B_LD = BK * BN / kThreads;
-  constexpr int SA = BM + 1, SB = BN + 1;  // padded LDS rows
-  static_assert(BM % 32 == 0 && BN % 32 == 0 && BK % 4 == 0, "tile shape");
-  static_assert((BM * BK) % kThreads == 0 && (BK * BN) % kThreads == 0, "load split");
-  // per-thread coordinates of the elements it stages (fixed for the kernel)
-  // A NoTrans: k fastest (rows of op(A) contiguous along k); A Trans: m fastest
-  constexpr int A_STEP = TA ? kThreads / BM : kThreads / BK;  // row (or k) step per e
-  constexpr int B_STEP = TB ? kThreads / BK : kThreads / BN;
-  static_assert(TA ? kThreads % BM == 0 : kThreads % BK == 0, "A staging layout");
-  static_assert(TB ? kThreads % BK == 0 : kThreads % BN == 0, "B staging layout");
-
-  __shared__ T sA[2][BK][SA];
-  __shared__ T sB[2][BK][SB];
-  __shared__ tamx_gemm_pair s_pairs[kPairWin];
-  __shared__ int s_item;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = uni(tid >> 6);
-  const int wr = (wave >> 1) * (BM / 2), wc = (wave & 1) * (BN / 2);
-  const int lrow = lane & 15, lk = lane >> 4;
-  // A staging: element e is (row a_r + e*A_STEP or a_r, k a_k or a_k + e*A_STEP)
-  const int a_fast = TA ? tid % BM : tid % BK;  // m (Trans) or k (NoTrans)
-  const int a_slow = TA ? tid / BM : tid / BK;  // k (Trans) or m (NoTrans)
-  const int b_fast = TB ? tid % BK : tid % BN;  // k (Trans) or n (NoTrans)
-  const int b_slow = TB ? tid / BK : tid / BN;  // n (Trans) or k (NoTrans)
-
-  int64_t ntasks64 = p.counters[TAMX_WL_NTASKS];
-  // same validation as the planner: an invalid list has no items
-  if (ntasks64 < 0 || ntasks64 > p.task_capacity || p.counters[TAMX_WL_STATUS] != 0)
-    ntasks64 = 0;
-  const int32_t ntasks = uni((int32_t)ntasks64);
-  const int32_t total = uni(p.prefix[ntasks > 0 ? ntasks : 0]);
-
-  // Items are split into kRegions contiguous ranges, one per XCD: a
-  // workgroup drains the range of its own XCD (workgroups are dispatched to
-  // XCDs round-robin by blockIdx), so the macro tiles of one result tile,
-  // which share A and B panels, run on one L2; then it steals from the
-  // other ranges in turn.
-  const int home = blockIdx.x % kRegions;
-  int tried = 0;  // used by thread 0 only
-  for (;;) {
-    if (tid == 0) {
-      int it = -1;
-      while (tried < kRegions) {
-        const int r = (home + tried) % kRegions;
-        const int rb = (int)((int64_t)total * r / kRegions);
-        const int re = (int)((int64_t)total * (r + 1) / kRegions);
-        const int q = rb < re ? atomicAdd(&p.queue[r], 1) : 0;
-        if (rb + q < re) {
-          it = rb + q;
-          break;
-        }
-        ++tried;
-      }
-      s_item = it;
-    }
-    __syncthreads();
-    const int item = uni(s_item);
-    __syncthreads();
-    if (item < 0) break;
-
-    int lo;
-    if (total <= p.item_capacity) {
-      lo = uni(p.item_task[item]);
-    } else {
-      lo = 0;
-      int hi = ntasks - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (uni(p.prefix[mid]) <= item) lo = mid; else hi = mid - 1;
-      }
-    }
-    const tamx_gemm_task* tp = p.tasks + lo;
-    const int64_t t_coff = uni64(tp->c_off);
-    const int t_m = uni(tp->m), t_n = uni(tp->n), t_ldc = uni(tp->ldc);
-    const int t_pb = uni(tp->pair_begin), t_pc = uni(tp->pair_count);
-    const int t_flags = uni(tp->flags);
-    const int local = item - uni(p.prefix[lo]);
-    const int tiles_n = (t_n + BN - 1) / BN;
-    const int r0 = (local / tiles_n) * BM, c0 = (local % tiles_n) * BN;
-    const int mrem = min(BM, t_m - r0), nrem = min(BN, t_n - c0);
-    // a wave whose quadrant lies wholly outside a ragged edge skips its
-    // MFMAs, and inside a quadrant every 16 x 16 block that does (the
-    // conditions are uniform across the wave)
-    const bool wave_active = (wr < mrem) && (wc < nrem);
-    bool row_on[FM], col_on[FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i) row_on[i] = wr + i * 16 < mrem;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) col_on[j] = wc + j * 16 < nrem;
-
-    acc_t acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
-
-    // Per-pair staging state of the load cursor: the thread's element
-    // offsets (relative to the pair's tile) at k0 = 0, and whether its
-    // row/column is inside the tile.
-    int64_t a_base = 0, b_base = 0;
-    int a_lda = 0, b_ldb = 0, pk = 0;
-    int64_t a_off[A_LD], b_off[B_LD];
-    bool a_ok[A_LD], b_ok[B_LD];
-    // pair descriptors of the task: in LDS when they fit, else read from
-    // global memory
-    const int pend = t_pb + t_pc;
-    const bool pairs_in_lds = t_pc <= kPairWin;
-    if (pairs_in_lds && t_pc > 0) {
-      const int32_t* src = reinterpret_cast<const int32_t*>(p.pairs + t_pb);
-      int32_t* dst = reinterpret_cast<int32_t*>(s_pairs);
-      constexpr int W = sizeof(tamx_gemm_pair) / sizeof(int32_t);
-      for (int x = tid; x < t_pc * W; x += kThreads) dst[x] = src[x];
-    }
-    __syncthreads();
-    auto set_pair = [&](int pi) {
-      if (pairs_in_lds) {
-        const tamx_gemm_pair& pp = s_pairs[pi - t_pb];
-        a_base = uni64(pp.a_off);
-        b_base = uni64(pp.b_off);
-        pk = uni(pp.k);
-        a_lda = uni(pp.lda);
-        b_ldb = uni(pp.ldb);
-      } else {
-        const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
-        a_base = uni64(pp.a_off);
-        b_base = uni64(pp.b_off);
-        pk = uni(pp.k);
-        a_lda = uni(pp.lda);
-        b_ldb = uni(pp.ldb);
-      }
-#pragma unroll
-      for (int e = 0; e < A_LD; ++e) {
-        const int slow = a_slow + e * A_STEP;
-        if (TA) {  // (k = slow, m = a_fast)
-          a_ok[e] = a_fast < mrem;
-          a_off[e] = (int64_t)slow * a_lda + r0 + a_fast;
-        } else {  // (m = slow, k = a_fast)
-          a_ok[e] = slow < mrem;
-          a_off[e] = (int64_t)(r0 + slow) * a_lda + a_fast;
-        }
-      }
-#pragma unroll
-      for (int e = 0; e < B_LD; ++e) {
-        const int slow = b_slow + e * B_STEP;
-        if (TB) {  // (n = slow, k = b_fast)
-          b_ok[e] = slow < nrem;
-          b_off[e] = (int64_t)(c0 + slow) * b_ldb + b_fast;
-        } else {  // (k = slow, n = b_fast)
-          b_ok[e] = b_fast < nrem;
-          b_off[e] = (int64_t)slow * b_ldb + c0 + b_fast;
-        }
-      }
-    };
-
-    // load cursor (pair lp, offset lk0) runs two chunks ahead of compute
-    int lp = t_pb, lk0 = 0;
-    if (lp < pend) set_pair(lp);
-    auto advance = [&]() {
-      lk0 += BK;
-      if (lk0 >= pk) {
-        ++lp;
-        lk0 = 0;
-        if (lp < pend) set_pair(lp);
-      }
-    };
-    // loads of chunk lk0 of the cursor's pair; out-of-tile elements read a
-    // zero constant, so the loaded registers need no select and nothing
-    // waits for the loads before they are stored to LDS.  Once the cursor
-    // has passed the last pair the loads repeat its first chunk, unused:
-    // every step issues the same loads, which keeps the wait counts exact.
-    auto load_chunk = [&](T (&ra)[A_LD], T (&rb)[B_LD]) {
-      const int kc = min(BK, pk - lk0);
-      const T* __restrict__ a = p.A + a_base;
-      const T* __restrict__ b = p.B + b_base;
-      const int64_t a_k = TA ? (int64_t)lk0 * a_lda : lk0;
-      const int64_t b_k = TB ? lk0 : (int64_t)lk0 * b_ldb;
-#pragma unroll
-      for (int e = 0; e < A_LD; ++e) {
-        const int kk = TA ? a_slow + e * A_STEP : a_fast;
-        const bool ok = a_ok[e] && kk < kc;
-        ra[e] = *(ok ? tamx::global_ptr(a + a_off[e] + a_k) : tamx::global_ptr(zero_of<T>()));
-      }
-#pragma unroll
-      for (int e = 0; e < B_LD; ++e) {
-        const int kk = TB ? b_fast : b_slow + e * B_STEP;
-        const bool ok = b_ok[e] && kk < kc;
-        rb[e] = *(ok ? tamx::global_ptr(b + b_off[e] + b_k) : tamx::global_ptr(zero_of<T>()));
-      }
-      if (lp < pend) advance();
-    };
-    auto store_chunk = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD]) {
-#pragma unroll
-      for (int e = 0; e < A_LD; ++e) {
-        if (TA) sA[buf][a_slow + e * A_STEP][a_fast] = ra[e];
-        else sA[buf][a_fast][a_slow + e * A_STEP] = ra[e];
-      }
-#pragma unroll
-      for (int e = 0; e < B_LD; ++e) {
-        if (TB) sB[buf][b_fast][b_slow + e * B_STEP] = rb[e];
-        else sB[buf][b_slow + e * B_STEP][b_fast] = rb[e];
-      }
-    };
-    auto compute_chunk = [&](int buf) {
-      if (!wave_active) return;
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 4) {
-        T af[FM], bf[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            if (row_on[i] && col_on[j]) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
-      }
-    };
-
-    // Pipeline: registers R0/R1 hold chunks c+1 and c+2 in flight while
-    // chunk c is multiplied out of LDS buffer c % 2.  One barrier per chunk.
-    T ra0[A_LD], rb0[B_LD], ra1[A_LD], rb1[B_LD];
-    if (lp < pend) {
-      load_chunk(ra0, rb0);                    // chunk 0
-      const bool have1 = lp < pend;
-      load_chunk(ra1, rb1);                    // chunk 1 (if have1)
-      store_chunk(0, ra0, rb0);
-      __syncthreads();
-      bool more = have1;  // chunk c + 1 exists
-      for (;;) {
-        // even step: compute buffer 0, chunk c+1 is in R1
-        bool next2 = lp < pend;
-        load_chunk(ra0, rb0);                  // chunk c + 2 -> R0 (if next2)
-        compute_chunk(0);
-        if (!more) break;
-        store_chunk(1, ra1, rb1);
-        __syncthreads();
-        more = next2;
-        // odd step: compute buffer 1, chunk c+1 is in R0
-        next2 = lp < pend;
-        load_chunk(ra1, rb1);                  // chunk c + 2 -> R1 (if next2)
-        compute_chunk(1);
-        if (!more) break;
-        store_chunk(0, ra0, rb0);
-        __syncthreads();
-        more = next2;
-      }
-    }
-
-    const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
-    T* __restrict__ c = p.C + t_coff;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = wr + i * 16 + M::row(lk, v);
-          const int col = wc + j * 16 + lrow;
-          if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
-                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
-            T val = p.alpha * acc[i][j][v];
-            if (beta1) val += c[idx];
-            c[idx] = val;
-          }
-        }
-  }
-
-  if (tid == 0) {
-    __threadfence();
-    const int done = atomicAdd(&p.queue[kRegions], 1);
-    if (done == (int)gridDim.x - 1) {
-      // last workgroup out: every grab has happened, reset for the next launch
-      for (int r = 0; r <= kRegions; ++r) atomicExch(&p.queue[r], 0);
-    }
-  }
-}
-
-struct TileCfg {
-  int bm, bn;
-};
-
-template <typename T, int BM, int BN, int BK, bool TA, bool TB>
-int launch_cfg(tamx_context* ctx, tamx_worklist* wl, const GemmParams<T>& prm,
-               hipStream_t s, int wgs_per_cu) {
-  auto kern = grouped_gemm_kernel<T, BM, BN, BK, TA, TB>;
-  static std::mutex mu;
-  static std::unordered_map<int, int> occ_cache;
-  int occ = 0;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    auto it = occ_cache.find(ctx->device);
-    if (it == occ_cache.end()) {
-      int o = 0;
-      TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, kern, kThreads, 0));
-      occ_cache[ctx->device] = o > 0 ? o : 1;
-      occ = occ_cache[ctx->device];
-    } else {
-      occ = it->second;
-    }
-  }
-  const int grid = ctx->num_cus * (wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ);
-  if (int rc = tamx::plan_items(wl, BM, BN, s)) return rc;
-  kern<<<grid, kThreads, 0, s>>>(prm);
-  TAMX_LAUNCH_CHECK();
-  return TAMX_OK;
-}
-
-template <typename T, int BM, int BN, int BK>
-int launch_ops(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
-               const GemmParams<T>& prm, hipStream_t s, int wgs) {
-  if (!op_a && !op_b) return launch_cfg<T, BM, BN, BK, false, false>(ctx, wl, prm, s, wgs);
-  if (!op_a && op_b) return launch_cfg<T, BM, BN, BK, false, true>(ctx, wl, prm, s, wgs);
-  if (op_a && !op_b) return launch_cfg<T, BM, BN, BK, true, false>(ctx, wl, prm, s, wgs);
-  return launch_cfg<T, BM, BN, BK, true, true>(ctx, wl, prm, s, wgs);
-}
-
-// Macro-tile shapes compiled into the library (BM, BN, BK).
-constexpr int kConfigs[][3] = {{64, 64, 16}, {64, 64, 32}, {128, 128, 16},
-                               {128, 64, 16}, {64, 128, 16}, {32, 32, 16},
-                               {32, 32, 32}, {128, 64, 32}, {64, 128, 32}};
-
-template <typename T>
-int grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
-                 T alpha, const T* A, const T* B, T* C, void* stream,
-                 const tamx_gemm_config* cfg) {
-  TAMX_REQUIRE(wl != nullptr, "null work list");
-  TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be TAMX_NO_TRANS or TAMX_TRANS");
-  TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be TAMX_NO_TRANS or TAMX_TRANS");
-  if (int rc = tamx::activate(ctx)) return rc;
-  GemmParams<T> prm{A, B, C, wl->tasks, wl->pairs, wl->counters,
-                    wl->item_prefix, wl->item_task, wl->queue,
-                    wl->task_capacity, wl->item_capacity, alpha};
-  hipStream_t s = tamx::as_stream(stream);
-  const int bm = cfg ? cfg->bm : 64, bn = cfg ? cfg->bn : 64, bk = cfg ? cfg->bk : 16;
-  const int w = cfg ? cfg->wgs_per_cu : 0;
-#define TAMX_CFG(BM_, BN_, BK_) \
-  if (bm == BM_ && bn == BN_ && bk == BK_) return launch_ops<T, BM_, BN_, BK_>(ctx, wl, op_a, op_b, prm, s, w);
-  TAMX_CFG(64, 64, 16)
-  TAMX_CFG(64, 64, 32)
-  TAMX_CFG(128, 128, 16)
-  TAMX_CFG(128, 64, 16)
-  TAMX_CFG(64, 128, 16)
-  TAMX_CFG(32, 32, 16)
-  TAMX_CFG(32, 32, 32)
-  TAMX_CFG(128, 64, 32)
-  TAMX_CFG(64, 128, 32)
-#undef TAMX_CFG
-  tamx::set_error("unsupported macro tile: see tamx_gemm_config_get");
-  return TAMX_ERR_UNSUPPORTED;
-}
-
 }  // namespace
 
 int tamx::plan_items(tamx_worklist* wl, int bm, int bn, hipStream_t s) {
@@ -531,26 +92,26 @@ int tamx_grouped_gemm_f64_config(tamx_context* ctx, tamx_worklist* wl, int op_a,
                                  int op_b, double alpha, const double* A,
                                  const double* B, double* C,
                                  const tamx_gemm_config* cfg, void* stream) {
-  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
+  return grouped_gemm<double, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
 }
 
 int tamx_grouped_gemm_f32_config(tamx_context* ctx, tamx_worklist* wl, int op_a,
                                  int op_b, float alpha, const float* A,
                                  const float* B, float* C,
                                  const tamx_gemm_config* cfg, void* stream) {
-  return grouped_gemm<float>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
+  return grouped_gemm<float, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg);
 }
 
 int tamx_grouped_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, double alpha, const double* A,
                           const double* B, double* C, void* stream) {
-  return grouped_gemm<double>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
+  return grouped_gemm<double, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
 }
 
 int tamx_grouped_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                           int op_b, float alpha, const float* A,
                           const float* B, float* C, void* stream) {
-  return grouped_gemm<float>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
+  return grouped_gemm<float, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, nullptr);
 }
 
 }  // extern "C"

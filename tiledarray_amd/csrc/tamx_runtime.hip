@@ -100,6 +100,17 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int precisi
                                int op_a, int op_b, const void* A, const void* B,
                                const void* C, int reps, int flags,
                                tamx_gemm_config* best, float* best_ms, void* stream) {
+  return tamx_autotune_grouped_gemm_gather(ctx, wl, precision, op_a, op_b, A, nullptr, B,
+                                           nullptr, C, reps, flags, best, best_ms, stream);
+}
+
+int tamx_autotune_grouped_gemm_gather(tamx_context* ctx, tamx_worklist* wl, int precision,
+                                      int op_a, int op_b, const void* A,
+                                      const tamx_gather_plan* a_gather, const void* B,
+                                      const tamx_gather_plan* b_gather, const void* C,
+                                      int reps, int flags, tamx_gemm_config* best,
+                                      float* best_ms, void* stream) {
+  const bool gathered = a_gather != nullptr || b_gather != nullptr;
   TAMX_REQUIRE(wl != nullptr && best != nullptr, "null argument");
   TAMX_REQUIRE(precision == TAMX_PREC_F64 || precision == TAMX_PREC_F32,
                "autotuning covers TAMX_PREC_F64 and TAMX_PREC_F32");
@@ -119,7 +130,8 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int precisi
   TAMX_REQUIRE(A != nullptr && B != nullptr && (C != nullptr || !sum.accumulates),
                "null operand");
   const TuneKey key{ctx->device, precision, op_a * 2 + op_b, bucket(sum.mean_m),
-                    bucket(sum.mean_n), bucket(sum.mean_k), bucket((double)sum.ntasks), 0};
+                    bucket(sum.mean_n), bucket(sum.mean_k), bucket((double)sum.ntasks),
+                    (a_gather ? 1 : 0) + (b_gather ? 2 : 0)};
   if (!(flags & TAMX_TUNE_FORCE)) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -143,6 +155,16 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int precisi
     return TAMX_OK;
   };
   auto launch = [&](const tamx_gemm_config& cfg) -> int {
+    if (gathered && precision == TAMX_PREC_F64)
+      return tamx_grouped_gemm_gather_f64(ctx, wl, op_a, op_b, 1.0,
+                                          static_cast<const double*>(A), a_gather,
+                                          static_cast<const double*>(B), b_gather,
+                                          static_cast<double*>(tmp.p), &cfg, s);
+    if (gathered)
+      return tamx_grouped_gemm_gather_f32(ctx, wl, op_a, op_b, 1.0f,
+                                          static_cast<const float*>(A), a_gather,
+                                          static_cast<const float*>(B), b_gather,
+                                          static_cast<float*>(tmp.p), &cfg, s);
     if (precision == TAMX_PREC_F64)
       return tamx_grouped_gemm_f64_config(ctx, wl, op_a, op_b, 1.0,
                                           static_cast<const double*>(A),
@@ -160,19 +182,28 @@ int tamx_autotune_grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int precisi
     tamx_gemm_config_get(c, &cfg);
     if (!(flags & TAMX_TUNE_NO_WARMUP)) {
       if (int rc = seed()) return rc;
-      if (int rc = launch(cfg)) return rc;
+      const int rc = launch(cfg);
+      if (rc == TAMX_ERR_UNSUPPORTED && gathered) continue;  // not compiled gathered
+      if (rc) return rc;
     }
     float total = 0.0f;
+    bool skipped = false;
     for (int r = 0; r < reps; ++r) {
       if (int rc = seed()) return rc;
       TAMX_CHECK_HIP(hipEventRecord(e0.e, s));
-      if (int rc = launch(cfg)) return rc;
+      const int rc = launch(cfg);
+      if (rc == TAMX_ERR_UNSUPPORTED && gathered) {
+        skipped = true;
+        break;
+      }
+      if (rc) return rc;
       TAMX_CHECK_HIP(hipEventRecord(e1.e, s));
       TAMX_CHECK_HIP(hipEventSynchronize(e1.e));
       float ms = 0.0f;
       TAMX_CHECK_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
       total += ms;
     }
+    if (skipped) continue;
     const float mean = total / reps;
     if (best_t < 0.0f || mean < best_t) {
       best_t = mean;

@@ -101,17 +101,29 @@ class Graph:
             pass
 
 
-def autotune(ctx, wl, dtype, op_a, op_b, A, B, C, reps=3, force=False, warmup=True):
+def autotune(ctx, wl, dtype, op_a, op_b, A, B, C, reps=3, force=False, warmup=True,
+             a_gather=None, b_gather=None):
     """Fastest compiled tile configuration for the built work list `wl`
-    (tamx_autotune_grouped_gemm); returns ((bm, bn, bk, wgs_per_cu), ms)."""
+    (tamx_autotune_grouped_gemm, or its gathered form when a GatherPlan is
+    given); returns ((bm, bn, bk, wgs_per_cu), ms)."""
     prec = _lib.PREC_F64 if dtype == torch.float64 else _lib.PREC_F32
     cfg = _lib.GemmConfig()
     ms = ctypes.c_float()
     flags = (_lib.TUNE_FORCE if force else 0) | (0 if warmup else _lib.TUNE_NO_WARMUP)
-    call("tamx_autotune_grouped_gemm", ctx.ptr, wl.ptr, prec, int(op_a), int(op_b),
-         A.data_ptr(), B.data_ptr(), C.data_ptr(), int(reps), flags, ctypes.byref(cfg),
-         ctypes.byref(ms), ctx.stream())
+    if a_gather is None and b_gather is None:
+        call("tamx_autotune_grouped_gemm", ctx.ptr, wl.ptr, prec, int(op_a), int(op_b),
+             A.data_ptr(), B.data_ptr(), C.data_ptr(), int(reps), flags, ctypes.byref(cfg),
+             ctypes.byref(ms), ctx.stream())
+    else:
+        call("tamx_autotune_grouped_gemm_gather", ctx.ptr, wl.ptr, prec, int(op_a), int(op_b),
+             A.data_ptr(), _plan_ptr(a_gather), B.data_ptr(), _plan_ptr(b_gather),
+             C.data_ptr(), int(reps), flags, ctypes.byref(cfg), ctypes.byref(ms),
+             ctx.stream())
     return (cfg.bm, cfg.bn, cfg.bk, cfg.wgs_per_cu), ms.value
+
+
+def _plan_ptr(plan):
+    return None if plan is None else plan.ptr
 
 
 class WorkList:
@@ -193,6 +205,26 @@ class PermutePlan:
 
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class GatherPlan:
+    """Device index map of a permuted view's tiles (tamx_gather_plan): the
+    grouped GEMM reads the view's stored tiles through it."""
+
+    def __init__(self, ctx, maps):
+        nt, nr, nk, rext, rstr, kext, kstr = maps
+        p = ctypes.c_void_p()
+        call("tamx_gather_plan_create", ctx.ptr, nt, nr, nk, rext.ctypes.data, rstr.ctypes.data,
+             kext.ctypes.data, kstr.ctypes.data, ctypes.byref(p))
+        self.ptr = p
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tamx_gather_plan_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
 
 
 class _SplitOperand:
@@ -344,7 +376,7 @@ class Contraction:
 
     def __init__(self, ctx, target, A, left, B, right, alpha=1.0, threshold=FLOAT_EPS,
                  pair_threshold=0.0, precision="f64", tile=None, permute_operands=True,
-                 fuse_permute=True):
+                 fuse_permute=None):
         self.ctx = ctx
         self.alpha = float(alpha)
         self.precision = precision
@@ -352,18 +384,31 @@ class Contraction:
         self.plan = plan = plan_contraction(left, right, target)
         self.threshold = float(np.float32(threshold))
         dev = A.pool.device
-        # operands in GEMM layout.  A split precision reads a `general`
-        # operand through an index map while splitting it (no permuted
-        # copy) unless fuse_permute is off.
-        fuse = bool(fuse_permute) and precision.startswith("bf16x")
+        # operands in GEMM layout.  A `general` operand is either permuted
+        # into a copy (as the reference permutes on read) or read through an
+        # index map of its stored tiles, with no copy.  fuse_permute=None
+        # picks the map for the limb split (it stages every element once
+        # anyway) and the copy for the FP64 / FP32 GEMM, whose gathered
+        # loads cost about 10% of its rate on the ladder workloads, more
+        # than the permute pass; True forces the map (saving the copy's
+        # memory), False the copy.  The FP32 two-limb GEMM always copies.
+        if fuse_permute is None:
+            fuse = precision.startswith("bf16x")
+        else:
+            fuse = bool(fuse_permute) and precision != "f32x2"
         self.A_src, self.B_src = A, B
 
-        def gemm_layout(X, permtype, perm):
+        def gemm_layout(X, permtype, perm, inner):
             if permtype != 3:
                 return X
-            return permuted_view(X, perm) if fuse else permute_array(ctx, X, perm)
-        self.A = gemm_layout(A, plan.left_permtype, plan.left_perm[:plan.left_rank])
-        self.B = gemm_layout(B, plan.right_permtype, plan.right_perm[:plan.right_rank])
+            n_in = int(sum(inner))
+            if fuse and 1 <= n_in <= 4 and 1 <= len(inner) - n_in <= 4:
+                return permuted_view(X, perm)
+            return permute_array(ctx, X, perm)
+        self.A = gemm_layout(A, plan.left_permtype, plan.left_perm[:plan.left_rank],
+                             [plan.left_gemm_is_inner[d] for d in range(plan.left_rank)])
+        self.B = gemm_layout(B, plan.right_permtype, plan.right_perm[:plan.right_rank],
+                             [plan.right_gemm_is_inner[d] for d in range(plan.right_rank)])
         self.permute_operands = permute_operands
         Ag, Bg = self.A, self.B
         l_inner = [bool(plan.left_gemm_is_inner[d]) for d in range(plan.left_rank)]
@@ -402,6 +447,17 @@ class Contraction:
         self.pair_threshold = float(pair_threshold)
         self._desc = self._make_desc()
         self.nslices = 0
+        # gathered operands of the FP64 / FP32 GEMM and the load orientation
+        # their maps read best along (the op passed for them)
+        self.a_gather = self.b_gather = None
+        self.gemm_op_a, self.gemm_op_b = self.op_a, self.op_b
+        if precision in ("f64", "f32"):
+            if hasattr(self.A, "_view_src"):
+                self.a_gather = GatherPlan(ctx, self._gather_maps(self.A, False))
+                self.gemm_op_a = 0 if self._k_fastest(self.A, False) else 1
+            if hasattr(self.B, "_view_src"):
+                self.b_gather = GatherPlan(ctx, self._gather_maps(self.B, True))
+                self.gemm_op_b = 1 if self._k_fastest(self.B, True) else 0
         if precision.startswith("bf16x"):
             if self.dtype != torch.float64:
                 raise ValueError("split precision takes FP64 operands")
@@ -442,8 +498,37 @@ class Contraction:
 
     def _split_plan_general(self, X, is_right):
         """Split plan of a permuted view: limb rows are the operand's outer
-        (non-contracted) modes, limb k its contracted modes, both in GEMM
-        order, each mapped to its stride in the stored source tile."""
+        (non-contracted) modes, limb k its contracted modes (_gather_maps)."""
+        nt, nr, nk, rext, rstr, kext, kstr = self._gather_maps(X, is_right)
+        offs = np.ascontiguousarray(X.offsets, np.int64)
+        p = ctypes.c_void_p()
+        call("tamx_split_plan_create_general", self.ctx.ptr, self.nslices, nt, offs.ctypes.data,
+             nr, nk, rext.ctypes.data, rstr.ctypes.data, kext.ctypes.data, kstr.ctypes.data,
+             ctypes.byref(p))
+        n = int(lib().tamx_split_plan_limb_elems(p))
+        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
+        return _SplitOperand(p, limbs, nt)
+
+    def _gemm_modes(self, is_right):
+        plan = self.plan
+        rank = plan.right_rank if is_right else plan.left_rank
+        inner = [bool((plan.right_gemm_is_inner if is_right else plan.left_gemm_is_inner)[d])
+                 for d in range(rank)]
+        return ([d for d in range(rank) if not inner[d]], [d for d in range(rank) if inner[d]])
+
+    def _k_fastest(self, X, is_right):
+        """Whether the stored stride-1 mode of a permuted view is its last
+        contracted mode (else loads run along the rows)."""
+        row_modes, k_modes = self._gemm_modes(is_right)
+        rank = len(row_modes) + len(k_modes)
+        inv = np.argsort(X._view_perm)
+        last = [dp for dp in range(rank) if inv[dp] == rank - 1][0]
+        return last != row_modes[-1]
+
+    def _gather_maps(self, X, is_right):
+        """Per-tile index maps of a permuted view: the extents and stored
+        strides of its outer modes (rows of op(X), GEMM order) and of its
+        contracted modes (k), by the view's tile ordinals."""
         plan = self.plan
         rank = plan.right_rank if is_right else plan.left_rank
         inner = [bool((plan.right_gemm_is_inner if is_right else plan.left_gemm_is_inner)[d])
@@ -454,11 +539,11 @@ class Contraction:
         row_modes = [d for d in range(rank) if not inner[d]]
         k_modes = [d for d in range(rank) if inner[d]]
         if not (1 <= len(row_modes) <= 4 and 1 <= len(k_modes) <= 4):
-            raise ValueError("fused permute-on-split covers 1 to 4 outer and inner modes")
+            raise ValueError("an index map covers 1 to 4 outer and 1 to 4 inner modes")
         nt = X.trange.ntiles
-        rext = np.zeros((nt, len(row_modes)), np.int32)
+        rext = np.ones((nt, len(row_modes)), np.int32)
         rstr = np.zeros((nt, len(row_modes)), np.int64)
-        kext = np.zeros((nt, len(k_modes)), np.int32)
+        kext = np.ones((nt, len(k_modes)), np.int32)
         kstr = np.zeros((nt, len(k_modes)), np.int64)
         for t in range(nt):
             e = sext[src_of[t]]
@@ -469,14 +554,7 @@ class Contraction:
                 rext[t, w], rstr[t, w] = e[inv[dp]], strides[inv[dp]]
             for w, dp in enumerate(k_modes):
                 kext[t, w], kstr[t, w] = e[inv[dp]], strides[inv[dp]]
-        offs = np.ascontiguousarray(X.offsets, np.int64)
-        p = ctypes.c_void_p()
-        call("tamx_split_plan_create_general", self.ctx.ptr, self.nslices, nt, offs.ctypes.data,
-             len(row_modes), len(k_modes), rext.ctypes.data, rstr.ctypes.data, kext.ctypes.data,
-             kstr.ctypes.data, ctypes.byref(p))
-        n = int(lib().tamx_split_plan_limb_elems(p))
-        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
-        return _SplitOperand(p, limbs, nt)
+        return nt, len(row_modes), len(k_modes), rext, rstr, kext, kstr
 
     def split_step(self, maxabs=None):
         """Split both operands into BF16 limbs (part of the split path).
@@ -545,8 +623,9 @@ class Contraction:
             return tile, ms.value
         if self.precision not in ("f64", "f32"):
             raise ValueError("autotuning covers the f64, f32 and BF16-limb grouped GEMMs")
-        tile, ms = autotune(self.ctx, self.wl, self.C.dtype, self.op_a, self.op_b, self.A.pool,
-                            self.B.pool, self.C, reps=reps, force=force, warmup=warmup)
+        tile, ms = autotune(self.ctx, self.wl, self.C.dtype, self.gemm_op_a, self.gemm_op_b,
+                            self.A.pool, self.B.pool, self.C, reps=reps, force=force,
+                            warmup=warmup, a_gather=self.a_gather, b_gather=self.b_gather)
         self.set_tile(tile)
         return tile, ms
 
@@ -574,7 +653,13 @@ class Contraction:
         s = self.ctx.stream()
         A, B, C = self.A.pool.data_ptr(), self.B.pool.data_ptr(), self.C.data_ptr()
         cfg = None if self.tile is None else ctypes.byref(self.tile_cfg)
-        if self.precision == "f64":
+        if self.a_gather is not None or self.b_gather is not None:
+            f64 = self.precision == "f64"
+            call("tamx_grouped_gemm_gather_f64" if f64 else "tamx_grouped_gemm_gather_f32",
+                 self.ctx.ptr, self.wl.ptr, self.gemm_op_a, self.gemm_op_b,
+                 ctypes.c_double(self.alpha) if f64 else ctypes.c_float(self.alpha),
+                 A, _plan_ptr(self.a_gather), B, _plan_ptr(self.b_gather), C, cfg, s)
+        elif self.precision == "f64":
             call("tamx_grouped_gemm_f64_config", self.ctx.ptr, self.wl.ptr, self.op_a, self.op_b,
                  ctypes.c_double(self.alpha), A, B, C, cfg, s)
         elif self.precision == "f32":
