@@ -26,6 +26,7 @@ METRIC = "nonzero-tile TFLOP/s, FP64 vs split-precision MFMA at stated rel. erro
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_INT8_TOPS = 5000.0
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes, each scaled by what the counter reads for a plain copy
@@ -218,13 +219,15 @@ def main():
     c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, tile=tile,
                       precision=args.precision)
     used_tile = (autotune(args, c, make_step(c))
-                 if args.precision in ("f64", "f32") or args.precision.startswith("bf16") else None)
+                 if args.precision in ("f64", "f32") or args.precision.startswith(("bf16", "i8"))
+                 else None)
     ms, gemm_ms = time_steps(c, make_step(c), args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
     value = flops / (ms * 1e-3) / 1e12
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if args.precision == "f64" else PEAK_BF16_TFLOPS
+    peak = (PEAK_FP64_TFLOPS if args.precision == "f64"
+            else PEAK_INT8_TOPS if args.precision.startswith("i8") else PEAK_BF16_TFLOPS)
 
     out = {
         "metric": METRIC,
@@ -369,11 +372,13 @@ def bench_ladder(args):
             c.gemm_step()
 
     used_tile = (autotune(args, c, step, reps=1)
-                 if args.precision == "f64" or args.precision.startswith("bf16") else None)
+                 if args.precision == "f64" or args.precision.startswith(("bf16", "i8"))
+                 else None)
     ms, gemm_ms = time_steps(c, step, args.steps, args.warmup)
     cnt = c.counters()
     flops = cnt["flops"]
-    peak = PEAK_FP64_TFLOPS if args.precision == "f64" else PEAK_BF16_TFLOPS
+    peak = (PEAK_FP64_TFLOPS if args.precision == "f64"
+            else PEAK_INT8_TOPS if args.precision.startswith("i8") else PEAK_BF16_TFLOPS)
     out = base_line(args, flops / (ms * 1e-3) / 1e12, ms,
                     "f64" if args.precision == "f64" else args.precision,
                     f"ladder R(a,b,i,j)=T(c,d,i,j)*V(a,c,b,d), o={args.o}, v={args.v}, "

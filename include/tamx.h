@@ -387,15 +387,36 @@ int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntile
                                    const int64_t* host_row_str,
                                    const int32_t* host_k_ext,
                                    const int64_t* host_k_str, tamx_split_plan** plan);
+/* Limb types.  TAMX_LIMB_BF16: the BF16 limbs above.  TAMX_LIMB_I8: INT8
+ * limbs, a = 2^(sigma-7) * sum_s l_s 2^(-7 s) + r with |l_0| <= 127,
+ * |l_s| <= 64 after it and |r| <= 2^(sigma - 7*nslices - 1) (sigma as above,
+ * one more where the tile max would round to 128), multiplied on the INT8
+ * MFMA with one exact INT32 accumulator per level s + t (an integer-slice
+ * scheme in the manner of Ozaki et al.); limbs are laid out like the BF16
+ * ones with K padded to 64 and one byte per limb. */
+#define TAMX_LIMB_BF16 0
+#define TAMX_LIMB_I8 1
+int tamx_split_plan_create_typed(tamx_context* ctx, int limb_type, int nslices,
+                                 int is_right, int op, int64_t ntiles,
+                                 const int64_t* host_offsets, const int32_t* host_rows,
+                                 const int32_t* host_cols, tamx_split_plan** plan);
+int tamx_split_plan_create_general_typed(tamx_context* ctx, int limb_type, int nslices,
+                                         int64_t ntiles, const int64_t* host_offsets,
+                                         int nrow_modes, int nk_modes,
+                                         const int32_t* host_row_ext,
+                                         const int64_t* host_row_str,
+                                         const int32_t* host_k_ext,
+                                         const int64_t* host_k_str, tamx_split_plan** plan);
+int tamx_split_plan_limb_type(const tamx_split_plan* plan);
 int tamx_split_plan_destroy(tamx_split_plan* plan);
 /* Limb count of a plan (0 for a null plan). */
 int tamx_split_plan_nslices(const tamx_split_plan* plan);
-/* BF16 elements the limb pool of a plan needs. */
+/* Limbs (BF16 or INT8 elements) the limb pool of a plan needs. */
 int64_t tamx_split_plan_limb_elems(const tamx_split_plan* plan);
 /* Copies the per-tile scale exponents to the host (synchronous). */
 int tamx_split_plan_sigmas(const tamx_split_plan* plan, int32_t* host_sigma);
-/* Writes the limbs of every tile of `pool` (device BF16 buffer of
- * tamx_split_plan_limb_elems elements). */
+/* Writes the limbs of every tile of `pool` (device buffer of
+ * tamx_split_plan_limb_elems BF16 or INT8 elements, by the plan's type). */
 int tamx_split_f64(tamx_context* ctx, const tamx_split_plan* plan,
                    const double* pool, void* limbs, void* stream);
 /* As tamx_split_f64 with each tile's max |a| given (device double, by the
@@ -408,15 +429,18 @@ int tamx_split_f64_premax(tamx_context* ctx, const tamx_split_plan* plan,
  * s + t < nslices on BF16 MFMA with FP32 accumulation (the s + t = 0 and
  * s + t = 1 sums exact, the lower levels together), folded into FP64 every
  * 128 k (cross-limb sums, the 2^(sigma_A + sigma_B) tile scaling and the C
- * accumulation in FP64).  Same work list and beta/transpose semantics as
- * tamx_grouped_gemm_f64. */
+ * accumulation in FP64).  INT8 limbs: the products on INT8 MFMA, one exact
+ * INT32 accumulator per level, folded at pair ends (at least every 4096 k).
+ * Both plans must have the same limb type and count.  Same work list and
+ * beta/transpose semantics as tamx_grouped_gemm_f64. */
 int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             const tamx_split_plan* a_plan, const void* a_limbs,
                             const tamx_split_plan* b_plan, const void* b_limbs,
                             double* C, void* stream);
 /* The same with a macro tile: bm x bn of 64 x 64 (the default, any limb
  * count), 128 x 64 or 64 x 128 (2 to 4 limbs; one wave per SIMD, a quarter
- * fewer limb bytes loaded per result).  bk and wgs_per_cu are ignored. */
+ * fewer limb bytes loaded per result; BF16 limbs only).  bk and wgs_per_cu
+ * are ignored. */
 int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double alpha,
                                    const tamx_split_plan* a_plan, const void* a_limbs,
                                    const tamx_split_plan* b_plan, const void* b_limbs,

@@ -294,3 +294,141 @@ def test_f32x2_error_bound(ctx):
     ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
     err = rel_fro(R.to_dense(), ref)
     assert err <= 1e-6, err
+
+
+# ------------------------------------------------------------- INT8 limbs --
+# relative Frobenius error bounds for INT8 limbs of 7 bits: a numpy
+# simulation of the same expansion on U[-1,1] 512^3 products (64-wide tiles)
+# gives 9.5e-7 / 8.3e-9 / 7.1e-11 / 6.0e-13 at 3 / 4 / 5 / 6 limbs (the
+# INT32 levels are exact, so no accumulator floor); the bounds leave ~30x
+I8_TOL = {3: 3e-5, 4: 3e-7, 5: 2e-9, 6: 2e-11}
+
+
+@pytest.mark.parametrize("nslices", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("is_right,op", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_split_i8_limbs_closed_form(ctx, nslices, is_right, op):
+    # the INT8 limbs of one tile against their definition:
+    #   |l_0| <= 127, |l_s| <= 64 after it,
+    #   |a - 2^(sigma-7) sum_s l_s 2^-7s| <= 2^(sigma - 7S - 1),
+    #   K padded to 64, padding zero
+    import ctypes
+    from tiledarray_amd import _lib
+    rows, cols = 37, 45
+    rng = np.random.default_rng(800 + nslices)
+    a = rng.uniform(-1, 1, (rows, cols)) * 3.7 * 2.0 ** rng.integers(-3, 4, (rows, cols))
+    pool = torch.as_tensor(a.reshape(-1), device="cuda")
+    plan = ctypes.c_void_p()
+    off = np.zeros(1, np.int64)
+    r32 = np.array([rows], np.int32)
+    c32 = np.array([cols], np.int32)
+    E.call("tamx_split_plan_create_typed", ctx.ptr, _lib.LIMB_I8, nslices, is_right, op, 1,
+           off.ctypes.data, r32.ctypes.data, c32.ctypes.data, ctypes.byref(plan))
+    try:
+        assert _lib.lib().tamx_split_plan_limb_type(plan) == _lib.LIMB_I8
+        n = _lib.lib().tamx_split_plan_limb_elems(plan)
+        limbs = torch.full((n,), 99, dtype=torch.uint8, device="cuda")
+        E.call("tamx_split_f64", ctx.ptr, plan, pool.data_ptr(), limbs.data_ptr(), ctx.stream())
+        torch.cuda.synchronize()
+        sig = np.zeros(1, np.int32)
+        E.call("tamx_split_plan_sigmas", plan, sig.ctypes.data)
+    finally:
+        _lib.lib().tamx_split_plan_destroy(plan)
+    sigma = int(sig[0])
+    trans = (op == 1) if not is_right else (op == 0)
+    logical = a.T if trans else a
+    m, k = logical.shape
+    mp, kp = -(-m // 16) * 16, -(-k // 64) * 64
+    L = limbs.cpu().numpy().view(np.int8).astype(np.float64)[:nslices * mp * kp]
+    L = L.reshape(nslices, mp, kp)
+    assert np.abs(L[0]).max() <= 127 and np.abs(L[1:]).max() <= 64
+    assert np.array_equal(L, np.rint(L))
+    assert not L[:, m:, :].any() and not L[:, :, k:].any()
+    w = 2.0 ** (-7 * np.arange(nslices))
+    rec = 2.0 ** (sigma - 7) * np.tensordot(w, L[:, :m, :k], axes=1)
+    assert np.abs(logical - rec).max() <= 2.0 ** (sigma - 7 * nslices - 1)
+    # sigma: the tile max in [2^(sigma-1), 2^sigma), or just below 2^(sigma-1)
+    # when it would round to 128 in the first limb
+    mx = np.abs(a).max()
+    assert mx < 2.0 ** sigma and (mx >= 2.0 ** (sigma - 1) or mx * 2.0 ** (8 - sigma) > 127)
+
+
+@pytest.mark.parametrize("nslices", [2, 3, 5, 6])
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+def test_split_i8_integer_exact(ctx, nslices, op_a, op_b):
+    rng = np.random.default_rng(400 + nslices * 4 + op_a * 2 + op_b)
+    trm = TiledRange1([0, 40, 130, 200])
+    trk = TiledRange1([0, 17, 64, 150, 151, 300])
+    trn = TiledRange1([0, 96, 97, 180])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
+    la = "k,m" if op_a else "m,k"
+    lb = "n,k" if op_b else "k,n"
+    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=f"i8x{nslices}")
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+@pytest.mark.parametrize("nslices", [3, 4, 5, 6])
+def test_split_i8_blocksparse_error_bound(ctx, nslices):
+    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=33)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8,
+                      precision=f"i8x{nslices}")
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    err = rel_fro(c.C.cpu().numpy()[mask], Cref[mask])
+    assert err <= I8_TOL[nslices], (nslices, err)
+
+
+def test_split_i8_long_pairs_fold(ctx):
+    # pairs longer than the 4096-k fold interval of the INT32 levels
+    rng = np.random.default_rng(12)
+    trm = TiledRange1([0, 70])
+    trk = TiledRange1([0, 5000, 9000])
+    trn = TiledRange1([0, 33])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, 0, 0, torch.float64, True, fill=1.0)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision="i8x3")
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+def test_split_i8_ladder_index_map(ctx):
+    # the general operand read through the index map, INT8 limbs: equal to
+    # the permuted copy's result bit for bit, and within the error bound
+    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=4)
+    cf = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision="i8x5")
+    cp = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision="i8x5",
+                       fuse_permute=False)
+    assert hasattr(cf.B, "_view_src") and hasattr(cp.B, "_perm_plan")
+    cf.run()
+    cp.run()
+    torch.cuda.synchronize()
+    assert torch.equal(cf.b_split.limbs, cp.b_split.limbs)
+    R = cf.result().to_dense()
+    np.testing.assert_array_equal(R, cp.result().to_dense())
+    ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
+    assert rel_fro(R, ref) <= LADDER_TOL[4]  # five INT8 limbs ~ four BF16 ones
+
+
+def test_split_i8_autotune_and_wide_tile_rejected(ctx):
+    from tiledarray_amd import _lib
+    A, B, _ = W.blocksparse_gemm(n=800, wlo=24, whi=96, seed=6)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="i8x4")
+    c.run()
+    torch.cuda.synchronize()
+    tile, ms = c.tune(reps=1, force=True)
+    assert tile[:2] == (64, 64) and ms > 0.0
+    c.set_tile((128, 64))
+    with pytest.raises(_lib.TamxError) as err:
+        c.gemm_step()
+    assert err.value.status == _lib.TAMX_ERR_UNSUPPORTED

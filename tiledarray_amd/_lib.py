@@ -18,6 +18,7 @@ WL_NTASKS, WL_NPAIRS, WL_CELEMS, WL_STATUS, WL_FLOPS, WL_NCOUNTERS = 0, 1, 2, 3,
 PLAN_MAX_RANK, PLAN_NAME_LEN = 8, 64
 PREC_F64, PREC_F32, PREC_BF16X3, PREC_BF16XS, PREC_F32X2 = 0, 1, 2, 3, 4
 TUNE_FORCE, TUNE_NO_WARMUP = 1, 2
+LIMB_BF16, LIMB_I8 = 0, 1
 
 
 class TamxError(RuntimeError):
@@ -141,6 +142,10 @@ SIGNATURES = {
     "tamx_autotune_grouped_gemm_split": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp,
                                          _vp],
     "tamx_split_plan_nslices": [_vp],
+    "tamx_split_plan_create_typed": [_vp, _i, _i, _i, _i, _i64, _vp, _vp, _vp, _vp],
+    "tamx_split_plan_create_general_typed": [_vp, _i, _i, _i64, _vp, _i, _i, _vp, _vp, _vp, _vp,
+                                             _vp],
+    "tamx_split_plan_limb_type": [_vp],
     "tamx_gather_plan_create": [_vp, _i64, _i, _i, _vp, _vp, _vp, _vp, _vp],
     "tamx_gather_plan_destroy": [_vp],
     "tamx_grouped_gemm_gather_f64": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -232,10 +232,11 @@ int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
   WorkListSummary sum;
   if (int rc = summarize(wl, s, sum)) return rc;
   const int nslices = tamx_split_plan_nslices(a_plan);
+  const int limb_type = tamx_split_plan_limb_type(a_plan);
   // the split GEMM's macro tiles: 64 x 64 for any limb count, 128 x 64 and
-  // 64 x 128 up to four limbs
+  // 64 x 128 up to four BF16 limbs
   std::vector<tamx_gemm_config> cands{{64, 64, 32, 0}};
-  if (nslices <= 4) {
+  if (nslices <= 4 && limb_type == TAMX_LIMB_BF16) {
     cands.push_back({128, 64, 32, 0});
     cands.push_back({64, 128, 32, 0});
   }
@@ -243,7 +244,8 @@ int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
   if (best_ms) *best_ms = 0.0f;
   if (sum.ntasks == 0) return TAMX_OK;
   TAMX_REQUIRE(C != nullptr || !sum.accumulates, "null C");
-  const TuneKey key{ctx->device, 100 + nslices, 0, bucket(sum.mean_m), bucket(sum.mean_n),
+  const TuneKey key{ctx->device, 100 + 10 * limb_type + nslices, 0, bucket(sum.mean_m),
+                    bucket(sum.mean_n),
                     bucket(sum.mean_k), bucket((double)sum.ntasks), 0};
   if (!(flags & TAMX_TUNE_FORCE)) {
     std::lock_guard<std::mutex> g(g_tune_mu);

@@ -182,12 +182,15 @@ __global__ __launch_bounds__(256) void split_max_kernel(
 
 // maxabs (when given) holds each tile's max |a|, else maxbits its bits
 __global__ void split_sigma_kernel(SplitTile* tiles, const unsigned long long* maxbits,
-                                   const double* maxabs, int64_t ntiles) {
+                                   const double* maxabs, int64_t ntiles, int i8) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const double m = maxabs ? maxabs[t] : __longlong_as_double((long long)maxbits[t]);
-  // x = a * 2^-sigma lies in (-1, 1): m < 2^(ilogb(m) + 1)
-  tiles[t].sigma = m > 0.0 ? ilogb(m) + 1 : 0;
+  // x = a * 2^-sigma lies in (-1, 1): m < 2^(ilogb(m) + 1); INT8 limbs
+  // also need |x| <= 127/128 so that rint(128 x) fits a signed byte
+  int sigma = m > 0.0 ? ilogb(m) + 1 : 0;
+  if (i8 && m > 0.0 && ldexp(m, 7 - sigma) > 127.0) ++sigma;
+  tiles[t].sigma = sigma;
 }
 
 // Writes the S limb planes of every 64 x 32 block of the limb layout.
@@ -231,18 +234,77 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
   }
 }
 
+// INT8 limbs: x = a 2^-sigma with |x| <= 127/128; l_0 = rint(128 x) (|l_0|
+// <= 127), x_1 = 128 x - l_0 (|x_1| <= 1/2); l_s = rint(128 x_s) (|l_s| <=
+// 64), x_{s+1} = 128 x_s - l_s.  Every step is exact in FP64, so
+// a = 2^(sigma-7) sum_s l_s 2^-7s + r with |r| <= 2^(sigma - 7 S - 1).  A
+// thread writes its 8 consecutive k of each limb as one 8-byte store.
+template <int S>
+__global__ __launch_bounds__(256) void split_limbs_i8_kernel(
+    const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
+    const int64_t* __restrict__ prefix, int64_t ntiles, uint64_t* __restrict__ limbs) {
+#pragma clang fp contract(off)
+  __shared__ double stg[kSplitBK][kSplitBR + 1];
+  const int64_t nblocks = prefix[ntiles];
+  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
+    const int64_t ti = find_tile(prefix, ntiles, item);
+    const SplitTile& t = tiles[ti];
+    int r0, k0;
+    block_of(t, item - prefix[ti], r0, k0);
+    double v[8];
+    load_block8(pool, t, r0, k0, stg, v);
+    if (r0 + r >= t.rows_pad) continue;
+    const int sigma = t.sigma;
+    uint64_t out[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[s] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double xs0 = ldexp(v[i], 7 - sigma);  // 128 * a * 2^-sigma
+      double l = rint(xs0);
+      out[0] |= (uint64_t)(uint8_t)(int8_t)(int)l << (8 * i);
+      double rem = xs0 - l;
+#pragma unroll
+      for (int s = 1; s < S; ++s) {
+        const double xs = rem * 128.0;
+        l = rint(xs);
+        out[s] |= (uint64_t)(uint8_t)(int8_t)(int)l << (8 * i);
+        rem = xs - l;
+      }
+    }
+    // in units of 8 limbs
+    const int64_t plane = ((int64_t)t.rows_pad * t.k_pad) >> 3;
+    const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
+#pragma unroll
+    for (int s = 0; s < S; ++s) limbs[o + s * plane] = out[s];
+  }
+}
+
 template <int S>
 void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_plan* p,
                   void* limbs) {
-  split_limbs_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles,
-                                             static_cast<bf8_t*>(limbs));
+  if (p->limb_type == TAMX_LIMB_I8)
+    split_limbs_i8_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles,
+                                                  static_cast<uint64_t*>(limbs));
+  else
+    split_limbs_kernel<S><<<grid, 256, 0, s>>>(pool, p->tiles, p->block_prefix, p->ntiles,
+                                               static_cast<bf8_t*>(limbs));
 }
 
 // ------------------------------------------------------------ the GEMM --
 
-template <int S>
-__global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
-  constexpr int NG = S >= 3 ? 3 : S;  // accumulators: P_0, P_1, P_T
+// I8: INT8 limbs (7 bits each) on the INT8 MFMA 16x16x64 with one exact
+// INT32 accumulator per level s + t; otherwise BF16 limbs as above.  Both
+// stage 16-byte units (8 BF16 or 16 INT8 limbs of one row), 4 units of a
+// row per chunk, so the staging and the LDS layout are shared.
+template <int S, bool I8>
+__global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitParams p) {
+  constexpr int NG = I8 ? S : (S >= 3 ? 3 : S);  // BF16: P_0, P_1, P_T; INT8: levels
+  constexpr int CK = I8 ? SBK8 : SBK;            // k per chunk
+  constexpr int USH = I8 ? 4 : 3;                // log2 limbs per 16-byte unit
+  constexpr int FOLD = I8 ? kFoldChunks8 : kFoldChunks;
+  using lvl_t = std::conditional_t<I8, i4_t, f4_t>;
   // LDS holds only the limb chunks (S * 16 KB); the grabbed item is passed
   // through the first word of sA while the buffers are idle between items
   __shared__ bf8_t sA[2][S][SBM * 4];
@@ -307,7 +369,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     const bool wave_active = (wr < mrem) && (wc < nrem);
 
     d4_t acc[2][2];
-    f4_t lvl[NG][2][2];
+    lvl_t lvl[NG][2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -317,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0.0f;
+          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
       }
 
     const int pend = t_pb + t_pc;
@@ -331,11 +393,11 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       const auto* A = tamx::scalar_ptr(p.ta) + at;
       const auto* B = tamx::scalar_ptr(p.tb) + bt;
       lkp = uni(A->k_pad);
-      // element offsets of this thread's staging row in plane 0, in units of 8
-      la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> 3;
-      lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> 3;
-      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> 3;
-      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> 3;
+      // limb offsets of this thread's staging row in plane 0, in 16-byte units
+      la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> USH;
+      lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> USH;
+      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
+      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
     };
     if (lp < pend) set_pair(lp);
     // Once the cursor has passed the last pair the loads repeat its first
@@ -343,14 +405,14 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
     // counts before the LDS stores exact (a conditional load would make the
     // compiler wait for all loads in flight).
     auto load_chunk = [&](bf8_t (&ra)[S], bf8_t (&rb)[S]) {
-      const int64_t q = (lk0 >> 3) + st_q;
+      const int64_t q = (lk0 >> USH) + st_q;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         ra[s] = p.LA[la_row + s * la_plane + q];
         rb[s] = p.LB[lb_row + s * lb_plane + q];
       }
       if (lp < pend) {
-        lk0 += SBK;
+        lk0 += CK;
         if (lk0 >= lkp) {
           ++lp;
           lk0 = 0;
@@ -382,13 +444,19 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
         for (int i = 0; i < NI; ++i) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
 #pragma unroll
         for (int t = 0; t < S - s; ++t) {
-          const int g = s + t < 2 ? s + t : 2;
+          const int g = I8 ? s + t : (s + t < 2 ? s + t : 2);
 #pragma unroll
           for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int j = 0; j < NJ; ++j)
-              lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
-                                                                     lvl[g][i][j], 0, 0, 0);
+            for (int j = 0; j < NJ; ++j) {
+              if constexpr (I8)
+                lvl[g][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                    __builtin_bit_cast(i4_t, af[i]), __builtin_bit_cast(i4_t, bf[j][t]),
+                    lvl[g][i][j], 0, 0, 0);
+              else
+                lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
+                                                                       lvl[g][i][j], 0, 0, 0);
+            }
         }
       }
     };
@@ -401,8 +469,9 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       else if (nj == 2) compute_blocks(buf, one{}, two{});
       else compute_blocks(buf, one{}, one{});
     };
-    // fold the FP32 accumulators of the pair being computed (scale
-    // 2^(sigma_A + sigma_B - 16))
+    // fold the level accumulators of the pair being computed (scale
+    // 2^(sigma_A + sigma_B - 16), BF16; 2^(sigma_A + sigma_B - 14) with level
+    // weights 2^-7g, INT8)
     auto fold = [&](double scale) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -413,8 +482,8 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
             double tsum = 0.0;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) {
-              tsum += (double)lvl[g][i][j][v];
-              lvl[g][i][j][v] = 0.0f;
+              tsum += I8 ? ldexp((double)lvl[g][i][j][v], -7 * g) : (double)lvl[g][i][j][v];
+              lvl[g][i][j][v] = 0;
             }
             acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
           }
@@ -428,15 +497,15 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
       const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
       const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
       ckp = uni(A->k_pad);
-      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - 16);
+      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - (I8 ? 14 : 16));
     };
     if (cp < pend) set_compute_pair(cp);
     // after computing one chunk: fold at a pair end or every kFoldChunks
     auto after_compute = [&]() {
       ++since_fold;
-      ck0 += SBK;
+      ck0 += CK;
       const bool pair_end = ck0 >= ckp;
-      if (pair_end || since_fold == kFoldChunks) {
+      if (pair_end || since_fold == FOLD) {
         if (wave_active) fold(cscale);
         since_fold = 0;
       }
@@ -527,10 +596,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitParams p) {
   }
 }
 
-template <int S>
+template <int S, bool I8>
 int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm,
                       hipStream_t s) {
-  auto kern = split_gemm_kernel<S>;
+  auto kern = split_gemm_kernel<S, I8>;
   int occ = 0;
   TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
   if (occ < 1) occ = 1;
@@ -826,7 +895,8 @@ int split_f64(tamx_context* ctx, const tamx_split_plan* plan, const double* pool
       TAMX_LAUNCH_CHECK();
     }
   }
-  split_sigma_kernel<<<tgrid, 256, 0, s>>>(plan->tiles, plan->maxbits, tile_maxabs, plan->ntiles);
+  split_sigma_kernel<<<tgrid, 256, 0, s>>>(plan->tiles, plan->maxbits, tile_maxabs, plan->ntiles,
+                                           plan->limb_type == TAMX_LIMB_I8 ? 1 : 0);
   TAMX_LAUNCH_CHECK();
   if (plan->nblocks > 0) {
     switch (plan->nslices) {
@@ -849,9 +919,20 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
                            int64_t ntiles, const int64_t* host_offsets,
                            const int32_t* host_rows, const int32_t* host_cols,
                            tamx_split_plan** out) {
+  return tamx_split_plan_create_typed(ctx, TAMX_LIMB_BF16, nslices, is_right, op, ntiles,
+                                      host_offsets, host_rows, host_cols, out);
+}
+
+int tamx_split_plan_create_typed(tamx_context* ctx, int limb_type, int nslices, int is_right,
+                                 int op, int64_t ntiles, const int64_t* host_offsets,
+                                 const int32_t* host_rows, const int32_t* host_cols,
+                                 tamx_split_plan** out) {
   TAMX_REQUIRE(out != nullptr, "null output pointer");
   *out = nullptr;
+  TAMX_REQUIRE(limb_type == TAMX_LIMB_BF16 || limb_type == TAMX_LIMB_I8,
+               "limb_type must be TAMX_LIMB_BF16 or TAMX_LIMB_I8");
   TAMX_REQUIRE(nslices >= 2 && nslices <= 6, "nslices must be in [2, 6]");
+  const int kq = limb_type == TAMX_LIMB_I8 ? SBK8 : SBK;  // k padding quantum
   TAMX_REQUIRE(op == 0 || op == 1, "op must be TAMX_NO_TRANS or TAMX_TRANS");
   TAMX_REQUIRE(ntiles >= 0, "negative tile count");
   TAMX_REQUIRE(ntiles == 0 || (host_offsets && host_rows && host_cols), "null tile table");
@@ -879,7 +960,7 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
     st.out_rows = transpose ? st.cols : st.rows;
     st.out_k = transpose ? st.rows : st.cols;
     st.rows_pad = (st.out_rows + 15) / 16 * 16;
-    st.k_pad = std::max(32, (st.out_k + 31) / 32 * 32);
+    st.k_pad = std::max(kq, (st.out_k + kq - 1) / kq * kq);
     st.limb_off = off;
     off += (int64_t)nslices * st.rows_pad * st.k_pad;
     max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
@@ -891,6 +972,7 @@ int tamx_split_plan_create(tamx_context* ctx, int nslices, int is_right, int op,
   plan->ctx = ctx;
   plan->ntiles = ntiles;
   plan->nslices = nslices;
+  plan->limb_type = limb_type;
   plan->limb_elems = off + (int64_t)kLimbSlackRows * max_kpad * nslices;
   plan->nblocks = blocks;
   hipError_t e = hipMalloc(&plan->tiles, std::max<size_t>(1, tiles.size()) * sizeof(SplitTile));
@@ -919,9 +1001,24 @@ int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntile
                                    const int32_t* host_row_ext, const int64_t* host_row_str,
                                    const int32_t* host_k_ext, const int64_t* host_k_str,
                                    tamx_split_plan** out) {
+  return tamx_split_plan_create_general_typed(ctx, TAMX_LIMB_BF16, nslices, ntiles, host_offsets,
+                                              nrow_modes, nk_modes, host_row_ext, host_row_str,
+                                              host_k_ext, host_k_str, out);
+}
+
+int tamx_split_plan_create_general_typed(tamx_context* ctx, int limb_type, int nslices,
+                                         int64_t ntiles, const int64_t* host_offsets,
+                                         int nrow_modes, int nk_modes,
+                                         const int32_t* host_row_ext,
+                                         const int64_t* host_row_str,
+                                         const int32_t* host_k_ext, const int64_t* host_k_str,
+                                         tamx_split_plan** out) {
   TAMX_REQUIRE(out != nullptr, "null output pointer");
   *out = nullptr;
+  TAMX_REQUIRE(limb_type == TAMX_LIMB_BF16 || limb_type == TAMX_LIMB_I8,
+               "limb_type must be TAMX_LIMB_BF16 or TAMX_LIMB_I8");
   TAMX_REQUIRE(nslices >= 2 && nslices <= 6, "nslices must be in [2, 6]");
+  const int kq = limb_type == TAMX_LIMB_I8 ? SBK8 : SBK;  // k padding quantum
   TAMX_REQUIRE(ntiles >= 0, "negative tile count");
   TAMX_REQUIRE(nrow_modes >= 1 && nrow_modes <= 4 && nk_modes >= 1 && nk_modes <= 4,
                "row and k modes must number 1 to 4 each");
@@ -959,7 +1056,7 @@ int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntile
     st.rows = st.out_rows;
     st.cols = st.out_k;
     st.rows_pad = (st.out_rows + 15) / 16 * 16;
-    st.k_pad = std::max(32, (st.out_k + 31) / 32 * 32);
+    st.k_pad = std::max(kq, (st.out_k + kq - 1) / kq * kq);
     st.limb_off = off;
     off += (int64_t)nslices * st.rows_pad * st.k_pad;
     max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
@@ -971,6 +1068,7 @@ int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntile
   plan->ctx = ctx;
   plan->ntiles = ntiles;
   plan->nslices = nslices;
+  plan->limb_type = limb_type;
   plan->limb_elems = off + (int64_t)kLimbSlackRows * max_kpad * nslices;
   plan->nblocks = blocks;
   hipError_t e = hipMalloc(&plan->tiles, std::max<size_t>(1, tiles.size()) * sizeof(SplitTile));
@@ -1001,6 +1099,10 @@ int tamx_split_plan_destroy(tamx_split_plan* plan) {
   if (plan->maxbits) (void)hipFree(plan->maxbits);
   delete plan;
   return TAMX_OK;
+}
+
+int tamx_split_plan_limb_type(const tamx_split_plan* plan) {
+  return plan ? plan->limb_type : TAMX_LIMB_BF16;
 }
 
 int tamx_split_plan_nslices(const tamx_split_plan* plan) {
@@ -1046,6 +1148,7 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                                    double* C, const tamx_gemm_config* cfg, void* stream) {
   TAMX_REQUIRE(wl && a_plan && b_plan && a_limbs && b_limbs, "null argument");
   TAMX_REQUIRE(a_plan->nslices == b_plan->nslices, "operands split into different limb counts");
+  TAMX_REQUIRE(a_plan->limb_type == b_plan->limb_type, "operands split into different limb types");
   if (int rc = tamx::activate(ctx)) return rc;
   SplitParams prm{static_cast<const bf8_t*>(a_limbs),
                   static_cast<const bf8_t*>(b_limbs),
@@ -1062,14 +1165,27 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                   wl->item_capacity,
                   alpha};
   hipStream_t s = tamx::as_stream(stream);
+  if (a_plan->limb_type == TAMX_LIMB_I8) {
+    if (cfg && !(cfg->bm == SBM && cfg->bn == SBN)) {
+      tamx::set_error("INT8-limb split GEMM macro tile: 64x64");
+      return TAMX_ERR_UNSUPPORTED;
+    }
+    switch (a_plan->nslices) {
+      case 2: return launch_split_gemm<2, true>(ctx, wl, prm, s);
+      case 3: return launch_split_gemm<3, true>(ctx, wl, prm, s);
+      case 4: return launch_split_gemm<4, true>(ctx, wl, prm, s);
+      case 5: return launch_split_gemm<5, true>(ctx, wl, prm, s);
+      default: return launch_split_gemm<6, true>(ctx, wl, prm, s);
+    }
+  }
   if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
     return launch_split_gemm_wide(a_plan->nslices, cfg->bm, cfg->bn, ctx, wl, prm, s);
   switch (a_plan->nslices) {
-    case 2: return launch_split_gemm<2>(ctx, wl, prm, s);
-    case 3: return launch_split_gemm<3>(ctx, wl, prm, s);
-    case 4: return launch_split_gemm<4>(ctx, wl, prm, s);
-    case 5: return launch_split_gemm<5>(ctx, wl, prm, s);
-    default: return launch_split_gemm<6>(ctx, wl, prm, s);
+    case 2: return launch_split_gemm<2, false>(ctx, wl, prm, s);
+    case 3: return launch_split_gemm<3, false>(ctx, wl, prm, s);
+    case 4: return launch_split_gemm<4, false>(ctx, wl, prm, s);
+    case 5: return launch_split_gemm<5, false>(ctx, wl, prm, s);
+    default: return launch_split_gemm<6, false>(ctx, wl, prm, s);
   }
 }
 

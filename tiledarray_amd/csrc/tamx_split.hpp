@@ -33,6 +33,7 @@ struct SplitTile {
 };
 
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -57,6 +58,11 @@ constexpr int kThreads = 256;
 constexpr int kRegions = 8;     // XCDs of an MI355X; the queue holds kRegions + 1 counters
 constexpr int SBK = 32;         // K per chunk (one BF16 MFMA 16x16x32)
 constexpr int kFoldChunks = 4;  // fold FP32 levels into FP64 every 128 k
+// INT8 limbs: K per chunk (one INT8 MFMA 16x16x64); the INT32 levels are
+// exact and folded at pair ends or every 64 chunks (4096 k), far inside
+// their range
+constexpr int SBK8 = 64;
+constexpr int kFoldChunks8 = 64;
 // rows of slack past the last tile of a limb pool: a macro tile may read up
 // to this many rows beyond a tile's padded rows (those rows only reach
 // results that are not stored)
@@ -88,7 +94,8 @@ struct tamx_split_plan {
   tamx_context* ctx = nullptr;
   int64_t ntiles = 0;
   int32_t nslices = 0;
-  int64_t limb_elems = 0;   // size of the limb pool in bf16 elements
+  int32_t limb_type = 0;    // TAMX_LIMB_BF16 or TAMX_LIMB_I8
+  int64_t limb_elems = 0;   // size of the limb pool in limbs (bf16 or int8)
   int64_t nblocks = 0;      // 64 x 32 blocks of the limb layouts over all tiles
   tamx::split::SplitTile* tiles = nullptr;  // device
   int64_t* block_prefix = nullptr;          // device, ntiles + 1

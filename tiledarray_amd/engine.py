@@ -393,7 +393,7 @@ class Contraction:
         # than the permute pass; True forces the map (saving the copy's
         # memory), False the copy.  The FP32 two-limb GEMM always copies.
         if fuse_permute is None:
-            fuse = precision.startswith("bf16x")
+            fuse = precision.startswith(("bf16x", "i8x"))
         else:
             fuse = bool(fuse_permute) and precision != "f32x2"
         self.A_src, self.B_src = A, B
@@ -447,6 +447,7 @@ class Contraction:
         self.pair_threshold = float(pair_threshold)
         self._desc = self._make_desc()
         self.nslices = 0
+        self.limb_type = _lib.LIMB_I8 if precision.startswith("i8x") else _lib.LIMB_BF16
         # gathered operands of the FP64 / FP32 GEMM and the load orientation
         # their maps read best along (the op passed for them)
         self.a_gather = self.b_gather = None
@@ -458,10 +459,10 @@ class Contraction:
             if hasattr(self.B, "_view_src"):
                 self.b_gather = GatherPlan(ctx, self._gather_maps(self.B, True))
                 self.gemm_op_b = 1 if self._k_fastest(self.B, True) else 0
-        if precision.startswith("bf16x"):
+        if precision.startswith(("bf16x", "i8x")):
             if self.dtype != torch.float64:
                 raise ValueError("split precision takes FP64 operands")
-            self.nslices = int(precision[5:])
+            self.nslices = int(precision.split("x")[-1])
             self.a_split = self._split_plan(self.A, False, self.op_a)
             self.b_split = self._split_plan(self.B, True, self.op_b)
         elif precision not in ("f64", "f32", "f32x2"):
@@ -489,12 +490,15 @@ class Contraction:
         cols = np.ascontiguousarray(cols, np.int32)
         offs = np.ascontiguousarray(X.offsets, np.int64)
         p = ctypes.c_void_p()
-        call("tamx_split_plan_create", self.ctx.ptr, self.nslices, 1 if is_right else 0,
-             int(op), offs.size, offs.ctypes.data, rows.ctypes.data, cols.ctypes.data,
-             ctypes.byref(p))
-        n = int(lib().tamx_split_plan_limb_elems(p))
-        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
-        return _SplitOperand(p, limbs, offs.size)
+        call("tamx_split_plan_create_typed", self.ctx.ptr, self.limb_type, self.nslices,
+             1 if is_right else 0, int(op), offs.size, offs.ctypes.data, rows.ctypes.data,
+             cols.ctypes.data, ctypes.byref(p))
+        return _SplitOperand(p, self._limb_pool(p, X), offs.size)
+
+    def _limb_pool(self, plan, X):
+        n = int(lib().tamx_split_plan_limb_elems(plan))
+        dtype = torch.uint8 if self.limb_type == _lib.LIMB_I8 else torch.bfloat16
+        return torch.zeros(max(1, n), dtype=dtype, device=X.pool.device)
 
     def _split_plan_general(self, X, is_right):
         """Split plan of a permuted view: limb rows are the operand's outer
@@ -502,12 +506,10 @@ class Contraction:
         nt, nr, nk, rext, rstr, kext, kstr = self._gather_maps(X, is_right)
         offs = np.ascontiguousarray(X.offsets, np.int64)
         p = ctypes.c_void_p()
-        call("tamx_split_plan_create_general", self.ctx.ptr, self.nslices, nt, offs.ctypes.data,
-             nr, nk, rext.ctypes.data, rstr.ctypes.data, kext.ctypes.data, kstr.ctypes.data,
-             ctypes.byref(p))
-        n = int(lib().tamx_split_plan_limb_elems(p))
-        limbs = torch.zeros(max(1, n), dtype=torch.bfloat16, device=X.pool.device)
-        return _SplitOperand(p, limbs, nt)
+        call("tamx_split_plan_create_general_typed", self.ctx.ptr, self.limb_type, self.nslices,
+             nt, offs.ctypes.data, nr, nk, rext.ctypes.data, rstr.ctypes.data,
+             kext.ctypes.data, kstr.ctypes.data, ctypes.byref(p))
+        return _SplitOperand(p, self._limb_pool(p, X), nt)
 
     def _gemm_modes(self, is_right):
         plan = self.plan
