@@ -56,7 +56,7 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--precision", default="f64", help="f64, f32x2 or bf16xS (S BF16 limbs)")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision runs")
-    p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,f32x2",
+    p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,i8x4,i8x5,f32x2",
                    help="split precisions measured beside the FP64 run")
     p.add_argument("--split-tile", default=None,
                    help="macro tile BMxBN of the split GEMM (64x64, 128x64 or 64x128)")

@@ -246,15 +246,18 @@ class Permutation {
   std::vector<int> p_;
 };
 
-enum class Precision { FP64, FP32, BF16_LIMBS };
+// BF16_LIMBS / INT8_LIMBS: FP64 operands split into nslices BF16 or INT8
+// limbs (tamx_split_plan_create_typed), products on BF16 or INT8 MFMA, sums
+// and C in FP64
+enum class Precision { FP64, FP32, BF16_LIMBS, INT8_LIMBS };
 
 struct ContractOptions {
   float threshold = default_threshold();  // result-shape threshold
   float pair_threshold = 0.0f;            // per-pair filter (off, as in TA)
   Precision precision = Precision::FP64;
-  int nslices = 5;                        // BF16 limbs for BF16_LIMBS
+  int nslices = 5;                        // limbs for BF16_LIMBS / INT8_LIMBS
   bool autotune = false;                  // pick the tile shape by timing (FP64/FP32)
-  // BF16_LIMBS: a `general` operand is split straight from its stored tiles
+  // limb precisions: a `general` operand is split straight from its stored tiles
   // through the permutation's index map (no permuted copy) unless set
   bool permute_copy = false;
 };
@@ -547,7 +550,10 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
   // operands whose layout is `general` are permuted (permute-on-read), or,
   // for the limb split, only their structure is, and the split reads the
   // stored tiles through the permutation's index map
-  const bool map = opt.precision == Precision::BF16_LIMBS && !opt.permute_copy;
+  const bool limbs = opt.precision == Precision::BF16_LIMBS ||
+                     opt.precision == Precision::INT8_LIMBS;
+  const int limb_type = opt.precision == Precision::INT8_LIMBS ? TAMX_LIMB_I8 : TAMX_LIMB_BF16;
+  const bool map = limbs && !opt.permute_copy;
   DistArray<T> Ap, Bp;
   const DistArray<T>* A = &A0;
   const DistArray<T>* B = &B0;
@@ -628,7 +634,7 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
   check(tamx_contraction_worklist(w.ctx(), &d, wl, w.stream()), "tamx_contraction_worklist");
   std::unique_ptr<tamx_split_plan, int (*)(tamx_split_plan*)> sa(nullptr, tamx_split_plan_destroy),
       sb(nullptr, tamx_split_plan_destroy);
-  DeviceBuffer<uint16_t> la_limbs, lb_limbs;
+  DeviceBuffer<uint8_t> la_limbs, lb_limbs;
   if (opt.precision == Precision::FP64 || opt.precision == Precision::FP32) {
     const bool f64 = std::is_same<T, double>::value;
     tamx_gemm_config cfg{};
@@ -693,11 +699,11 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
         }
       }
       tamx_split_plan* sp = nullptr;
-      check(tamx_split_plan_create_general(w.ctx(), opt.nslices, (int64_t)nt,
-                                           view.offsets().data(), (int)rows_m.size(),
-                                           (int)k_m.size(), rext.data(), rstr.data(),
-                                           kext.data(), kstr.data(), &sp),
-            "tamx_split_plan_create_general");
+      check(tamx_split_plan_create_general_typed(w.ctx(), limb_type, opt.nslices, (int64_t)nt,
+                                                 view.offsets().data(), (int)rows_m.size(),
+                                                 (int)k_m.size(), rext.data(), rstr.data(),
+                                                 kext.data(), kstr.data(), &sp),
+            "tamx_split_plan_create_general_typed");
       return sp;
     };
     std::vector<int32_t> ar, ac, br, bc;
@@ -708,20 +714,23 @@ void contract(World& w, DistArray<T>& result, const std::string& target, const D
       pa = general_plan(A0, *A, a_src, plan.left_perm, plan.left_gemm_is_inner,
                         plan.left_rank);
     else
-      check(tamx_split_plan_create(w.ctx(), opt.nslices, 0, plan.op_a, (int64_t)ar.size(),
-                                   A->offsets().data(), ar.data(), ac.data(), &pa),
-            "tamx_split_plan_create");
+      check(tamx_split_plan_create_typed(w.ctx(), limb_type, opt.nslices, 0, plan.op_a,
+                                         (int64_t)ar.size(), A->offsets().data(), ar.data(),
+                                         ac.data(), &pa),
+            "tamx_split_plan_create_typed");
     sa.reset(pa);
     if (map && B != &B0)
       pb = general_plan(B0, *B, b_src, plan.right_perm, plan.right_gemm_is_inner,
                         plan.right_rank);
     else
-      check(tamx_split_plan_create(w.ctx(), opt.nslices, 1, plan.op_b, (int64_t)br.size(),
-                                   B->offsets().data(), br.data(), bc.data(), &pb),
-            "tamx_split_plan_create");
+      check(tamx_split_plan_create_typed(w.ctx(), limb_type, opt.nslices, 1, plan.op_b,
+                                         (int64_t)br.size(), B->offsets().data(), br.data(),
+                                         bc.data(), &pb),
+            "tamx_split_plan_create_typed");
     sb.reset(pb);
-    la_limbs = DeviceBuffer<uint16_t>(std::max<int64_t>(1, tamx_split_plan_limb_elems(pa)));
-    lb_limbs = DeviceBuffer<uint16_t>(std::max<int64_t>(1, tamx_split_plan_limb_elems(pb)));
+    const int64_t lbytes = limb_type == TAMX_LIMB_I8 ? 1 : 2;  // bytes per limb
+    la_limbs = DeviceBuffer<uint8_t>(std::max<int64_t>(2, lbytes * tamx_split_plan_limb_elems(pa)));
+    lb_limbs = DeviceBuffer<uint8_t>(std::max<int64_t>(2, lbytes * tamx_split_plan_limb_elems(pb)));
     check(tamx_split_f64(w.ctx(), pa, reinterpret_cast<const double*>(a_data), la_limbs.get(),
                          w.stream()),
           "tamx_split_f64");

@@ -243,6 +243,21 @@ int main() {
     CHECK(dense(Rm) == dense(Rc), "index-map split differs from the permuted-copy split");
   });
 
+  run("split precision, 5 INT8 limbs, ladder through the index map", [&] {
+    auto T = make_array<double>(world, TiledRange{tv, tv, to, to}, rng, false);
+    auto V = make_array<double>(world, TiledRange{tv, tv, tv, tv}, rng, false);
+    DistArray<double> R(world, TiledRange{tv, tv, to, to});
+    ContractOptions opt;
+    opt.precision = Precision::INT8_LIMBS;
+    opt.nslices = 5;
+    R("a,b,i,j") = (T("c,d,i,j") * V("a,c,b,d")).with(opt);
+    std::vector<int64_t> er;
+    auto ref = einsum(dense(T), "c,d,i,j", extents(T), dense(V), "a,c,b,d", extents(V),
+                      "a,b,i,j", er);
+    const double err = rel_fro(dense(R), ref);
+    CHECK(err <= 3e-8, "relative error %g", err);
+  });
+
   run("autotuned tile shape, block-sparse", [&] {
     TiledRange1 tb{0, 37, 90, 150, 151, 260, 333};
     auto A = make_array<double>(world, TiledRange{tb, tb}, rng, true, 0.7);
