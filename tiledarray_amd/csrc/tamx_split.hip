@@ -420,6 +420,32 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
         }
       }
     };
+    // The same chunk loaded straight into LDS buffer `buf` (no registers):
+    // lane l of wave w writes LDS unit 64 w + l of each plane, which is the
+    // swizzled slot lds_unit(st_r, q) of row st_r = tid / 4 for the unit
+    // q = (tid % 4) ^ ((st_r / 2) % 4) of that row.  The caller waits for
+    // the loads (vmcnt) before the barrier that precedes reading them.
+    const int st_qs = st_q ^ ((st_r >> 1) & 3);
+    auto load_chunk_lds = [&](int buf) {
+      const int64_t q = (lk0 >> USH) + st_qs;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(p.LA + la_row + s * la_plane + q),
+            (__attribute__((address_space(3))) void*)(&sA[buf][s][wave * 64]), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(p.LB + lb_row + s * lb_plane + q),
+            (__attribute__((address_space(3))) void*)(&sB[buf][s][wave * 64]), 16, 0, 0);
+      }
+      if (lp < pend) {
+        lk0 += CK;
+        if (lk0 >= lkp) {
+          ++lp;
+          lk0 = 0;
+          if (lp < pend) set_pair(lp);
+        }
+      }
+    };
     auto store_chunk = [&](int buf, const bf8_t (&ra)[S], const bf8_t (&rb)[S]) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
@@ -521,19 +547,22 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     // Above four limbs the second register set would cost occupancy, so the
     // loads run one chunk ahead instead.
     bf8_t ra0[S], rb0[S], ra1[S], rb1[S];
-    if constexpr (S > 4) {
+    // Above four BF16 limbs (three INT8 limbs) a second register set would
+    // cost occupancy: the loads run one chunk ahead, straight into LDS.
+    constexpr bool ONE_AHEAD = I8 ? S > 3 : S > 4;
+    if constexpr (ONE_AHEAD) {
       if (lp < pend) {
-        load_chunk(ra0, rb0);
-        store_chunk(0, ra0, rb0);
+        load_chunk_lds(0);
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int buf = 0;
         for (;;) {
           const bool has_next = lp < pend;
-          load_chunk(ra0, rb0);
+          if (has_next) load_chunk_lds(buf ^ 1);  // chunk c+1 -> the idle buffer
           compute_chunk(buf);
           after_compute();
           if (!has_next) break;
-          store_chunk(buf ^ 1, ra0, rb0);
+          __builtin_amdgcn_s_waitcnt(0);
           __syncthreads();
           buf ^= 1;
         }
