@@ -508,7 +508,10 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
             double tsum = 0.0;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) {
-              tsum += I8 ? ldexp((double)lvl[g][i][j][v], -7 * g) : (double)lvl[g][i][j][v];
+              // (INT8: the level weight 2^-7g is exact, so the fma rounds
+              // once, as the sum would)
+              if constexpr (I8) tsum = fma((double)lvl[g][i][j][v], 1.0 / (double)(1LL << (7 * g)), tsum);
+              else tsum += (double)lvl[g][i][j][v];
               lvl[g][i][j][v] = 0;
             }
             acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
