@@ -54,7 +54,8 @@ def parse():
                    help="macro tile BMxBN[xBK[xWGS]] of the grouped GEMM, or 'auto' (autotuner)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-threads", type=int, default=0)
-    p.add_argument("--precision", default="f64", help="f64, f32x2 or bf16xS (S BF16 limbs)")
+    p.add_argument("--precision", default="f64",
+                   help="f64, f32x2, bf16xS or i8xS (S BF16 or INT8 limbs)")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision runs")
     p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,i8x4,i8x5,f32x2",
                    help="split precisions measured beside the FP64 run")

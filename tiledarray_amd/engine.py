@@ -372,6 +372,13 @@ class Contraction:
 
     Host work (planning, operand permutation plans, fusion metadata, result
     allocation) happens here; run() enqueues device work only.
+
+    precision: "f64" / "f32" (the operands' type on FP64 / FP32 MFMA),
+    "bf16xS" or "i8xS" (FP64 operands split into S BF16 or INT8 limbs,
+    2 <= S <= 6, products on BF16 / INT8 MFMA, sums and C in FP64) or
+    "f32x2" (two FP32 limbs).  fuse_permute: how a `general` operand is
+    brought into GEMM layout (None: an index map for the limb precisions, a
+    permuted copy otherwise; True: an index map; False: a copy).
     """
 
     def __init__(self, ctx, target, A, left, B, right, alpha=1.0, threshold=FLOAT_EPS,
@@ -611,7 +618,7 @@ class Contraction:
     def tune(self, reps=3, force=False, warmup=True):
         """Autotunes the tile configuration on the current work list (built
         by worklist_step, with the limbs split for a split precision) and
-        keeps it; FP64, FP32 and BF16-limb precisions."""
+        keeps it; FP64, FP32, BF16-limb and INT8-limb precisions."""
         if self.nslices:
             cfg = _lib.GemmConfig()
             ms = ctypes.c_float()
@@ -624,7 +631,7 @@ class Contraction:
             self.set_tile(tile)
             return tile, ms.value
         if self.precision not in ("f64", "f32"):
-            raise ValueError("autotuning covers the f64, f32 and BF16-limb grouped GEMMs")
+            raise ValueError("autotuning covers the f64, f32 and limb-split grouped GEMMs")
         tile, ms = autotune(self.ctx, self.wl, self.C.dtype, self.gemm_op_a, self.gemm_op_b,
                             self.A.pool, self.B.pool, self.C, reps=reps, force=force,
                             warmup=warmup, a_gather=self.a_gather, b_gather=self.b_gather)
