@@ -58,6 +58,17 @@ __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t it
   return lo;
 }
 
+// The contiguous range [b, e) of 64 x 32 blocks of this workgroup and the
+// tile of its first block: the tile table is searched once per workgroup,
+// then followed forward (tile t covers blocks [prefix[t], prefix[t+1])).
+__device__ inline void block_range(const int64_t* prefix, int64_t ntiles, int64_t& b,
+                                   int64_t& e, int64_t& ti) {
+  const int64_t n = prefix[ntiles];
+  b = n * blockIdx.x / gridDim.x;
+  e = n * (blockIdx.x + 1) / gridDim.x;
+  ti = b < e ? find_tile(prefix, ntiles, b) : 0;
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -157,9 +168,10 @@ __global__ __launch_bounds__(256) void split_max_kernel(
     unsigned long long* __restrict__ maxbits) {
   __shared__ double stg[kSplitBK][kSplitBR + 1];
   __shared__ double red[256];
-  const int64_t nblocks = prefix[ntiles];
-  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
-    const int64_t ti = find_tile(prefix, ntiles, item);
+  int64_t b, e, ti;
+  block_range(prefix, ntiles, b, e, ti);
+  for (int64_t item = b; item < e; ++item) {
+    while (prefix[ti + 1] <= item) ++ti;
     const SplitTile& t = tiles[ti];
     int r0, k0;
     block_of(t, item - prefix[ti], r0, k0);
@@ -200,10 +212,11 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
     const int64_t* __restrict__ prefix, int64_t ntiles, bf8_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
   __shared__ double stg[kSplitBK][kSplitBR + 1];
-  const int64_t nblocks = prefix[ntiles];
+  int64_t b, e, ti;
+  block_range(prefix, ntiles, b, e, ti);
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
-    const int64_t ti = find_tile(prefix, ntiles, item);
+  for (int64_t item = b; item < e; ++item) {
+    while (prefix[ti + 1] <= item) ++ti;
     const SplitTile& t = tiles[ti];
     int r0, k0;
     block_of(t, item - prefix[ti], r0, k0);
@@ -245,10 +258,11 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     const int64_t* __restrict__ prefix, int64_t ntiles, uint64_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
   __shared__ double stg[kSplitBK][kSplitBR + 1];
-  const int64_t nblocks = prefix[ntiles];
+  int64_t b, e, ti;
+  block_range(prefix, ntiles, b, e, ti);
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  for (int64_t item = blockIdx.x; item < nblocks; item += gridDim.x) {
-    const int64_t ti = find_tile(prefix, ntiles, item);
+  for (int64_t item = b; item < e; ++item) {
+    while (prefix[ti + 1] <= item) ++ti;
     const SplitTile& t = tiles[ti];
     int r0, k0;
     block_of(t, item - prefix[ti], r0, k0);
