@@ -137,3 +137,31 @@ def test_planner_rejects_bad_annotations(left, right, target):
     rc, _ = _plan(left, right, target)
     assert rc == _lib.TAMX_ERR_BAD_ARG if hasattr(_lib, "TAMX_ERR_BAD_ARG") else rc == 1
     assert lib().tamx_last_error()
+
+
+def test_plan_constructors_reject_bad_arguments_before_touching_a_device():
+    # argument checks come before any device call, so they run without a GPU
+    L = _lib.lib()
+    p = ctypes.c_void_p()
+    off = np.zeros(1, np.int64)
+    ext = np.array([4], np.int32)
+    st = np.array([1], np.int64)
+    # unknown limb type, limb count out of range
+    assert L.tamx_split_plan_create_typed(None, 7, 3, 0, 0, 1, off.ctypes.data, ext.ctypes.data,
+                                          ext.ctypes.data, ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
+    assert L.tamx_split_plan_create_typed(None, _lib.LIMB_I8, 7, 0, 0, 1, off.ctypes.data,
+                                          ext.ctypes.data, ext.ctypes.data,
+                                          ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
+    assert L.tamx_split_plan_create_general_typed(None, _lib.LIMB_BF16, 3, 1, off.ctypes.data, 5,
+                                                  1, ext.ctypes.data, st.ctypes.data,
+                                                  ext.ctypes.data, st.ctypes.data,
+                                                  ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
+    # gather plans: 1 to 4 modes each side
+    assert L.tamx_gather_plan_create(None, 1, 0, 1, ext.ctypes.data, st.ctypes.data,
+                                     ext.ctypes.data, st.ctypes.data,
+                                     ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
+    assert L.tamx_gather_plan_create(None, -1, 1, 1, ext.ctypes.data, st.ctypes.data,
+                                     ext.ctypes.data, st.ctypes.data,
+                                     ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
+    assert p.value is None
+    assert L.tamx_split_plan_limb_type(None) == _lib.LIMB_BF16
