@@ -438,9 +438,9 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             const tamx_split_plan* b_plan, const void* b_limbs,
                             double* C, void* stream);
 /* The same with a macro tile: bm x bn of 64 x 64 (the default, any limb
- * count), 128 x 64 or 64 x 128 (2 to 4 limbs; one wave per SIMD, a quarter
- * fewer limb bytes loaded per result; BF16 limbs only).  bk and wgs_per_cu
- * are ignored. */
+ * count), 128 x 64 or 64 x 128 (2 to 4 BF16 or 3 to 4 INT8 limbs; one wave
+ * per SIMD, a quarter fewer limb bytes loaded per result).  bk and
+ * wgs_per_cu are ignored. */
 int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double alpha,
                                    const tamx_split_plan* a_plan, const void* a_limbs,
                                    const tamx_split_plan* b_plan, const void* b_limbs,

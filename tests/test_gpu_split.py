@@ -84,20 +84,20 @@ def test_norms_with_max_and_premax_split(ctx):
 
 
 @pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
-@pytest.mark.parametrize("nslices", [2, 3, 4])
+@pytest.mark.parametrize("precision", ["bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_split_wide_tiles_integer_exact(ctx, tile, nslices, op_a, op_b):
+def test_split_wide_tiles_integer_exact(ctx, tile, precision, op_a, op_b):
     # the wide macro tiles (one wave per SIMD) on tiles wider and narrower
     # than one macro tile, ragged in both directions
-    rng = np.random.default_rng(900 + nslices * 4 + op_a * 2 + op_b)
+    nslices = int(precision.split("x")[-1])
+    rng = np.random.default_rng(900 + nslices * 4 + op_a * 2 + op_b + 50 * ("i8" in precision))
     trm = TiledRange1([0, 40, 170, 300, 301])
     trk = TiledRange1([0, 17, 64, 150, 151, 300])
     trn = TiledRange1([0, 96, 97, 250])
     A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
     la = "k,m" if op_a else "m,k"
     lb = "n,k" if op_b else "k,n"
-    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=f"bf16x{nslices}",
-                      tile=tile)
+    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=precision, tile=tile)
     c.run()
     torch.cuda.synchronize()
     Cref, npairs, c_off = oracle_contract(c, A, B)
@@ -106,13 +106,14 @@ def test_split_wide_tiles_integer_exact(ctx, tile, nslices, op_a, op_b):
 
 
 @pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
-def test_split_wide_tiles_match_default(ctx, tile):
+@pytest.mark.parametrize("precision", ["bf16x4", "i8x4"])
+def test_split_wide_tiles_match_default(ctx, tile, precision):
     # same limbs, same pair order: the wide kernel's result equals the
     # default kernel's bit for bit on random data
     A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=21)
     E.make_sparse(ctx, A, 1e-8)
     E.make_sparse(ctx, B, 1e-8)
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="bf16x4")
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision=precision)
     c.run()
     torch.cuda.synchronize()
     C0 = c.C.clone()
@@ -427,7 +428,13 @@ def test_split_i8_autotune_and_wide_tile_rejected(ctx):
     c.run()
     torch.cuda.synchronize()
     tile, ms = c.tune(reps=1, force=True)
-    assert tile[:2] == (64, 64) and ms > 0.0
+    assert tile[:2] in ((64, 64), (128, 64), (64, 128)) and ms > 0.0
+    # five INT8 limbs have the 64 x 64 tile only
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="i8x5")
+    c.run()
+    torch.cuda.synchronize()
+    tile, ms = c.tune(reps=1, force=True)
+    assert tile[:2] == (64, 64)
     c.set_tile((128, 64))
     with pytest.raises(_lib.TamxError) as err:
         c.gemm_step()

@@ -1214,10 +1214,9 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                   alpha};
   hipStream_t s = tamx::as_stream(stream);
   if (a_plan->limb_type == TAMX_LIMB_I8) {
-    if (cfg && !(cfg->bm == SBM && cfg->bn == SBN)) {
-      tamx::set_error("INT8-limb split GEMM macro tile: 64x64");
-      return TAMX_ERR_UNSUPPORTED;
-    }
+    if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
+      return launch_split_gemm_wide(TAMX_LIMB_I8, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
+                                    prm, s);
     switch (a_plan->nslices) {
       case 2: return launch_split_gemm<2, true>(ctx, wl, prm, s);
       case 3: return launch_split_gemm<3, true>(ctx, wl, prm, s);
@@ -1227,7 +1226,8 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
     }
   }
   if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
-    return launch_split_gemm_wide(a_plan->nslices, cfg->bm, cfg->bn, ctx, wl, prm, s);
+    return launch_split_gemm_wide(TAMX_LIMB_BF16, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
+                                  prm, s);
   switch (a_plan->nslices) {
     case 2: return launch_split_gemm<2, false>(ctx, wl, prm, s);
     case 3: return launch_split_gemm<3, false>(ctx, wl, prm, s);

@@ -84,8 +84,8 @@ __device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ ((r >> 1) & 3
 
 // The grouped split GEMM with a bm x bn macro tile of 128 x 64 or 64 x 128
 // for 2 to 4 limbs (tamx_split_wide.hip); plans its items and launches.
-int launch_split_gemm_wide(int nslices, int bm, int bn, tamx_context* ctx, tamx_worklist* wl,
-                           const SplitParams& prm, hipStream_t s);
+int launch_split_gemm_wide(int limb_type, int nslices, int bm, int bn, tamx_context* ctx,
+                           tamx_worklist* wl, const SplitParams& prm, hipStream_t s);
 
 }  // namespace split
 }  // namespace tamx

@@ -16,9 +16,15 @@ namespace {
 
 using namespace tamx::split;
 
-template <int S, int WI, int WJ>
+// I8: INT8 limbs on the INT8 MFMA with one exact INT32 accumulator per level
+// (as in split_gemm_kernel); otherwise BF16 limbs.
+template <int S, int WI, int WJ, bool I8>
 __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParams p) {
-  constexpr int NG = S >= 3 ? 3 : S;          // accumulators: P_0, P_1, P_T
+  constexpr int NG = I8 ? S : (S >= 3 ? 3 : S);  // BF16: P_0, P_1, P_T; INT8: levels
+  constexpr int CK = I8 ? SBK8 : SBK;            // k per chunk
+  constexpr int USH = I8 ? 4 : 3;                // log2 limbs per 16-byte unit
+  constexpr int FOLD = I8 ? kFoldChunks8 : kFoldChunks;
+  using lvl_t = std::conditional_t<I8, i4_t, f4_t>;
   constexpr int BM = 32 * WI, BN = 32 * WJ;   // macro tile (2 x 2 waves)
   constexpr int RA = BM / 64, RB = BN / 64;   // staged rows per thread per plane
   static_assert(RA >= 1 && RB >= 1, "macro tile below 64 rows");
@@ -88,7 +94,7 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
     const bool wave_active = ni > 0 && nj > 0;
 
     d4_t acc[WI][WJ];
-    f4_t lvl[NG][WI][WJ];
+    lvl_t lvl[NG][WI][WJ];
 #pragma unroll
     for (int i = 0; i < WI; ++i)
 #pragma unroll
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
 #pragma unroll
         for (int g = 0; g < NG; ++g)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0.0f;
+          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
       }
 
     const int pend = t_pb + t_pc;
@@ -111,20 +117,20 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
       const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
       const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
       lkp = uni(A->k_pad);
-      // element offsets in units of 8: plane stride, this thread's first
+      // limb offsets in 16-byte units: plane stride, this thread's first
       // staging row, and the 64-row step between its rows
-      la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> 3;
-      lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> 3;
-      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> 3;
-      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> 3;
-      la_step = lb_step = (int64_t)(64 / 8) * lkp;
+      la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> USH;
+      lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> USH;
+      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
+      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
+      la_step = lb_step = (int64_t)(64 >> USH) * lkp;
     };
     if (lp < pend) set_pair(lp);
     // Once the cursor has passed the last pair the loads repeat its first
     // chunk, unused: every step issues the same loads, which keeps the wait
     // counts before the LDS stores exact.
     auto load_chunk = [&](bf8_t (&ra)[S][RA], bf8_t (&rb)[S][RB]) {
-      const int64_t q = (lk0 >> 3) + st_q;
+      const int64_t q = (lk0 >> USH) + st_q;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
         for (int u = 0; u < RB; ++u) rb[s][u] = p.LB[lb_row + u * lb_step + s * lb_plane + q];
       }
       if (lp < pend) {
-        lk0 += SBK;
+        lk0 += CK;
         if (lk0 >= lkp) {
           ++lp;
           lk0 = 0;
@@ -166,14 +172,20 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
           if (i < ni) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
 #pragma unroll
         for (int t = 0; t < S - s; ++t) {
-          const int g = s + t < 2 ? s + t : 2;
+          const int g = I8 ? s + t : (s + t < 2 ? s + t : 2);
 #pragma unroll
           for (int i = 0; i < WI; ++i) {
             if (i < ni) {
 #pragma unroll
-              for (int j = 0; j < NJ; ++j)
-                lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
-                                                                       lvl[g][i][j], 0, 0, 0);
+              for (int j = 0; j < NJ; ++j) {
+                if constexpr (I8)
+                  lvl[g][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                      __builtin_bit_cast(i4_t, af[i]), __builtin_bit_cast(i4_t, bf[j][t]),
+                      lvl[g][i][j], 0, 0, 0);
+                else
+                  lvl[g][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j][t],
+                                                                         lvl[g][i][j], 0, 0, 0);
+              }
             }
           }
         }
@@ -204,8 +216,11 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
               double tsum = 0.0;
 #pragma unroll
               for (int g = NG - 1; g >= 0; --g) {
-                tsum += (double)lvl[g][i][j][v];
-                lvl[g][i][j][v] = 0.0f;
+                if constexpr (I8)
+                  tsum = fma((double)lvl[g][i][j][v], 1.0 / (double)(1LL << (7 * g)), tsum);
+                else
+                  tsum += (double)lvl[g][i][j][v];
+                lvl[g][i][j][v] = 0;
               }
               acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
             }
@@ -221,14 +236,14 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
       const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
       const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
       ckp = uni(A->k_pad);
-      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - 16);
+      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - (I8 ? 14 : 16));
     };
     if (cp < pend) set_compute_pair(cp);
     auto after_compute = [&]() {
       ++since_fold;
-      ck0 += SBK;
+      ck0 += CK;
       const bool pair_end = ck0 >= ckp;
-      if (pair_end || since_fold == kFoldChunks) {
+      if (pair_end || since_fold == FOLD) {
         if (wave_active) fold(cscale);
         since_fold = 0;
       }
@@ -290,9 +305,9 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
   }
 }
 
-template <int S, int WI, int WJ>
+template <int S, int WI, int WJ, bool I8>
 int launch_wide(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm, hipStream_t s) {
-  auto kern = split_gemm_wide_kernel<S, WI, WJ>;
+  auto kern = split_gemm_wide_kernel<S, WI, WJ, I8>;
   int occ = 0;
   TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
   if (occ < 1) occ = 1;
@@ -304,18 +319,26 @@ int launch_wide(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm, hi
 
 }  // namespace
 
-int tamx::split::launch_split_gemm_wide(int nslices, int bm, int bn, tamx_context* ctx,
-                                        tamx_worklist* wl, const SplitParams& prm,
-                                        hipStream_t s) {
-#define TAMX_WIDE(S_, BM_, BN_) \
-  if (nslices == S_ && bm == BM_ && bn == BN_) return launch_wide<S_, BM_ / 32, BN_ / 32>(ctx, wl, prm, s);
-  TAMX_WIDE(2, 128, 64)
-  TAMX_WIDE(3, 128, 64)
-  TAMX_WIDE(4, 128, 64)
-  TAMX_WIDE(2, 64, 128)
-  TAMX_WIDE(3, 64, 128)
-  TAMX_WIDE(4, 64, 128)
+int tamx::split::launch_split_gemm_wide(int limb_type, int nslices, int bm, int bn,
+                                        tamx_context* ctx, tamx_worklist* wl,
+                                        const SplitParams& prm, hipStream_t s) {
+#define TAMX_WIDE(I8_, S_, BM_, BN_)                            \
+  if ((limb_type == TAMX_LIMB_I8) == I8_ && nslices == S_ && bm == BM_ && bn == BN_) \
+    return launch_wide<S_, BM_ / 32, BN_ / 32, I8_>(ctx, wl, prm, s);
+  TAMX_WIDE(false, 2, 128, 64)
+  TAMX_WIDE(false, 3, 128, 64)
+  TAMX_WIDE(false, 4, 128, 64)
+  TAMX_WIDE(false, 2, 64, 128)
+  TAMX_WIDE(false, 3, 64, 128)
+  TAMX_WIDE(false, 4, 64, 128)
+  // (five INT8 limbs at these tiles would need more than 512 registers per
+  // lane; this compiler also fails on them)
+  TAMX_WIDE(true, 3, 128, 64)
+  TAMX_WIDE(true, 4, 128, 64)
+  TAMX_WIDE(true, 3, 64, 128)
+  TAMX_WIDE(true, 4, 64, 128)
 #undef TAMX_WIDE
-  tamx::set_error("split GEMM macro tile: 64x64 (any limb count), 128x64 or 64x128 (2 to 4 limbs)");
+  tamx::set_error("split GEMM macro tile: 64x64 (any limb count), 128x64 or 64x128 (2 to 4 "
+                  "BF16 or 3 to 4 INT8 limbs)");
   return TAMX_ERR_UNSUPPORTED;
 }
