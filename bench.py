@@ -282,11 +282,14 @@ def main():
         rn, rmax = float(np.linalg.norm(r)), float(np.abs(r).max())
         split = {}
         for prec in args.split.split(","):
+            limbs = int(prec.split("x")[-1])
+            wide_ok = (prec.startswith("bf16") and limbs <= 4) or (prec.startswith("i8")
+                                                                  and 3 <= limbs <= 4)
             st = (tuple(int(x) for x in args.split_tile.split("x"))
-                  if args.split_tile and prec.startswith("bf16") and int(prec[5:]) <= 4 else None)
+                  if args.split_tile and wide_ok else None)
             cs = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, precision=prec,
                                tile=st)
-            if st is None and args.tile == "auto" and prec.startswith("bf16"):
+            if st is None and args.tile == "auto" and prec.startswith(("bf16", "i8")):
                 make_step(cs)(None)
                 torch.cuda.synchronize()
                 st = tuple(cs.tune(reps=2)[0][:2])
