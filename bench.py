@@ -414,9 +414,10 @@ def bench_ladder(args):
                                     time_kernel(lambda: Y._perm_plan.run(V.pool, Y.pool), reps=3))
         del Y
     if c.nslices:
-        # FP64 read, S BF16 limbs written, for both operands
+        # FP64 read, S BF16 (2 B) or INT8 (1 B) limbs written, for both operands
         nel = (stored_bytes(T) + stored_bytes(c.B)) // 8
-        bw["split_limbs"] = (nel * (8 + 2 * c.nslices),
+        limb_bytes = 1 if args.precision.startswith("i8") else 2
+        bw["split_limbs"] = (nel * (8 + limb_bytes * c.nslices),
                              time_kernel(lambda: c.split_step(maxabs=(maxabs[id(T)], perm_max)),
                                          reps=3))
     out["bandwidth"] = bandwidth(bw)
