@@ -74,9 +74,7 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-// The 8 values of thread t's (row, 8 k) in block (r0, k0) of tile t_, zero
-// outside the tile.  A transposed tile (stored k x rows) is staged through
-// `stg` so the global reads run along the stored rows.
+// Offset of index x over n modes (extents ext, strides str, the last fastest).
 __device__ inline int64_t mixed_radix_offset(int x, int n, const int32_t* ext,
                                              const int64_t* str) {
   int64_t off = 0;
@@ -89,6 +87,9 @@ __device__ inline int64_t mixed_radix_offset(int x, int n, const int32_t* ext,
   return off;
 }
 
+// The 8 values of thread t's (row, 8 k) in block (r0, k0) of tile t_, zero
+// outside the tile.  Every layout is staged through `stg`, with the lanes
+// running along the stored rows, so a wave's global reads are contiguous.
 __device__ inline void load_block8(const double* __restrict__ pool, const SplitTile& t, int r0,
                                    int k0, double (*stg)[kSplitBR + 1], double (&v)[8]) {
   const int tid = threadIdx.x;
@@ -145,13 +146,21 @@ __device__ inline void load_block8(const double* __restrict__ pool, const SplitT
     for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
     __syncthreads();
   } else {
-    const int gr = r0 + r;
-    const int64_t row = t.src_off + (int64_t)gr * t.cols;
+    // stored rows are limb rows, k contiguous: the lanes run along k (a
+    // wave reads two rows of 32 consecutive k per load) and the block is
+    // turned into the thread layout through LDS
+    const int kk = tid & 31, gk = k0 + kk;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int gk = k0 + 8 * q + i;
-      v[i] = (gr < t.out_rows && gk < t.out_k) ? pool[row + gk] : 0.0;
+    for (int j = 0; j < 8; ++j) {
+      const int rr = (tid >> 5) + 8 * j, gr = r0 + rr;
+      double x = 0.0;
+      if (gk < t.out_k && gr < t.out_rows) x = pool[t.src_off + (int64_t)gr * t.cols + gk];
+      stg[kk][rr] = x;
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
+    __syncthreads();
   }
 }
 
