@@ -261,6 +261,19 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
 // 64), x_{s+1} = 128 x_s - l_s.  Every step is exact in FP64, so
 // a = 2^(sigma-7) sum_s l_s 2^-7s + r with |r| <= 2^(sigma - 7 S - 1).  A
 // thread writes its 8 consecutive k of each limb as one 8-byte store.
+//
+// Rounding: for |y| < 2^51, y + 1.5 * 2^52 lies in [2^52, 2^53), where the
+// spacing of doubles is 1, so the sum is y rounded to an integer (ties to
+// even, as rint), and because 1.5 * 2^52 is even and the exponent does not
+// change, the low word of the sum's bits is that integer in two's
+// complement.  So each limb costs one FMA (round), one subtraction (the
+// limb as a double) and one FMA (the exact remainder), with no conversion.
+constexpr double kRoundShift = 6755399441055744.0;  // 1.5 * 2^52
+
+__device__ inline uint32_t low_bytes4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  return (a & 0xffu) | ((b & 0xffu) << 8) | ((c & 0xffu) << 16) | (d << 24);
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
@@ -279,23 +292,27 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     load_block8(pool, t, r0, k0, stg, v);
     if (r0 + r >= t.rows_pad) continue;
     const int sigma = t.sigma;
-    uint64_t out[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) out[s] = 0;
+    uint32_t w[S][8];  // limb s of element i in the low byte of w[s][i]
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const double xs0 = ldexp(v[i], 7 - sigma);  // 128 * a * 2^-sigma
-      double l = rint(xs0);
-      out[0] |= (uint64_t)(uint8_t)(int8_t)(int)l << (8 * i);
+      double tr = xs0 + kRoundShift;
+      double l = tr - kRoundShift;  // rint(xs0)
+      w[0][i] = (uint32_t)__double2loint(tr);
       double rem = xs0 - l;
 #pragma unroll
       for (int s = 1; s < S; ++s) {
-        const double xs = rem * 128.0;
-        l = rint(xs);
-        out[s] |= (uint64_t)(uint8_t)(int8_t)(int)l << (8 * i);
-        rem = xs - l;
+        tr = __fma_rn(rem, 128.0, kRoundShift);  // 128 rem is exact
+        l = tr - kRoundShift;                     // rint(128 rem)
+        w[s][i] = (uint32_t)__double2loint(tr);
+        rem = __fma_rn(rem, 128.0, -l);           // exact: |128 rem - l| <= 1/2
       }
     }
+    uint64_t out[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      out[s] = (uint64_t)low_bytes4(w[s][0], w[s][1], w[s][2], w[s][3]) |
+               (uint64_t)low_bytes4(w[s][4], w[s][5], w[s][6], w[s][7]) << 32;
     // in units of 8 limbs
     const int64_t plane = ((int64_t)t.rows_pad * t.k_pad) >> 3;
     const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
