@@ -87,87 +87,120 @@ __device__ inline int64_t mixed_radix_offset(int x, int n, const int32_t* ext,
   return off;
 }
 
-// The 8 values of thread t's (row, 8 k) in block (r0, k0) of tile t_, zero
-// outside the tile.  Every layout is staged through `stg`, with the lanes
-// running along the stored rows, so a wave's global reads are contiguous.
-__device__ inline void load_block8(const double* __restrict__ pool, const SplitTile& t, int r0,
-                                   int k0, double (*stg)[kSplitBR + 1], double (&v)[8]) {
+// A block of 64 limb rows x 32 k is read with the lanes along the stored
+// rows (a transposed tile, or a general one whose last row mode is the
+// source's stride-1 mode: element j of thread tid is k (tid >> 6) + 4 j, row
+// tid & 63) or along k (element j: k tid & 31, row (tid >> 5) + 8 j), so a
+// wave's global reads are contiguous, and turned into the thread layout
+// (row tid >> 2, k 8 (tid & 3) .. + 8) through LDS.
+__device__ inline bool lanes_along_rows(const SplitTile& t) {
+  return t.gen_nr > 0 ? t.gr_str[t.gen_nr - 1] == 1 : t.transpose != 0;
+}
+
+// Global loads of thread tid's 8 elements of block (r0, k0), zero outside
+// the tile.
+__device__ inline void fetch_block8(const double* __restrict__ pool, const SplitTile& t, int r0,
+                                    int k0, double (&x)[8]) {
   const int tid = threadIdx.x;
-  const int r = tid >> 2, q = tid & 3;
   if (t.gen_nr > 0) {
-    // general layout, staged through LDS like a transposed tile: the lanes
-    // run along the rows when the last row mode is the source's stride-1
-    // mode, else along k, so consecutive lanes read consecutive addresses
-    const bool rows_fast = t.gr_str[t.gen_nr - 1] == 1;
-    if (rows_fast) {
-      const int rr = tid & 63, gr = r0 + rr;
+    if (lanes_along_rows(t)) {
+      const int gr = r0 + (tid & 63);
       const bool row_ok = gr < t.out_rows;
       const int64_t base =
           t.src_off + (row_ok ? mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str) : 0);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int kk = (tid >> 6) + 4 * j, gk = k0 + kk;
-        double x = 0.0;
-        if (row_ok && gk < t.out_k)
-          x = pool[base + mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str)];
-        stg[kk][rr] = x;
+        const int gk = k0 + (tid >> 6) + 4 * j;
+        x[j] = row_ok && gk < t.out_k
+                   ? pool[base + mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str)]
+                   : 0.0;
       }
     } else {
-      const int kk = tid & 31, gk = k0 + kk;
+      const int gk = k0 + (tid & 31);
       const bool k_ok = gk < t.out_k;
       const int64_t base =
           t.src_off + (k_ok ? mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str) : 0);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int rr = (tid >> 5) + 8 * j, gr = r0 + rr;
-        double x = 0.0;
-        if (k_ok && gr < t.out_rows)
-          x = pool[base + mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str)];
-        stg[kk][rr] = x;
+        const int gr = r0 + (tid >> 5) + 8 * j;
+        x[j] = k_ok && gr < t.out_rows
+                   ? pool[base + mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str)]
+                   : 0.0;
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
-    __syncthreads();
-    return;
-  }
-  if (t.transpose) {
+  } else if (t.transpose) {
+    const int gr = r0 + (tid & 63);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int kk = (tid >> 6) + 4 * j, rr = tid & 63;
-      const int gk = k0 + kk, gr = r0 + rr;
-      double x = 0.0;
-      if (gk < t.out_k && gr < t.out_rows) x = pool[t.src_off + (int64_t)gk * t.cols + gr];
-      stg[kk][rr] = x;
+      const int gk = k0 + (tid >> 6) + 4 * j;
+      x[j] = gk < t.out_k && gr < t.out_rows ? pool[t.src_off + (int64_t)gk * t.cols + gr] : 0.0;
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
-    __syncthreads();
   } else {
-    // stored rows are limb rows, k contiguous: the lanes run along k (a
-    // wave reads two rows of 32 consecutive k per load) and the block is
-    // turned into the thread layout through LDS
-    const int kk = tid & 31, gk = k0 + kk;
+    const int gk = k0 + (tid & 31);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int rr = (tid >> 5) + 8 * j, gr = r0 + rr;
-      double x = 0.0;
-      if (gk < t.out_k && gr < t.out_rows) x = pool[t.src_off + (int64_t)gr * t.cols + gk];
-      stg[kk][rr] = x;
+      const int gr = r0 + (tid >> 5) + 8 * j;
+      x[j] = gk < t.out_k && gr < t.out_rows ? pool[t.src_off + (int64_t)gr * t.cols + gk] : 0.0;
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
-    __syncthreads();
   }
+}
+
+// The fetched elements through LDS into the thread layout: v[i] is row
+// tid >> 2, k 8 (tid & 3) + i of the block.  Every thread of the block
+// calls it (two barriers).
+__device__ inline void stage_block8(const SplitTile& t, double (*stg)[kSplitBR + 1],
+                                    const double (&x)[8], double (&v)[8]) {
+  const int tid = threadIdx.x;
+  if (lanes_along_rows(t)) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) stg[(tid >> 6) + 4 * j][tid & 63] = x[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) stg[tid & 31][(tid >> 5) + 8 * j] = x[j];
+  }
+  __syncthreads();
+  const int r = tid >> 2, q = tid & 3;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
+  __syncthreads();
 }
 
 __device__ inline void block_of(const SplitTile& t, int64_t local, int& r0, int& k0) {
   const int nbk = t.k_pad / kSplitBK;
   r0 = (int)(local / nbk) * kSplitBR;
   k0 = (int)(local % nbk) * kSplitBK;
+}
+
+// Runs body(tile, tile index, r0, k0, v) over the workgroup's contiguous
+// range of blocks, v being thread tid's 8 elements (row tid >> 2, k
+// 8 (tid & 3) + i).  The loads of the next block are issued before the body
+// of the current one runs, so they are in flight during its arithmetic and
+// stores.  Every thread runs every body (the bodies may hold barriers).
+template <class Body>
+__device__ inline void for_each_block8(const double* __restrict__ pool,
+                                       const SplitTile* __restrict__ tiles,
+                                       const int64_t* __restrict__ prefix, int64_t ntiles,
+                                       double (*stg)[kSplitBR + 1], Body&& body) {
+  int64_t b, e, ti;
+  block_range(prefix, ntiles, b, e, ti);
+  if (b >= e) return;
+  while (prefix[ti + 1] <= b) ++ti;
+  int r0, k0;
+  block_of(tiles[ti], b - prefix[ti], r0, k0);
+  double x[8];
+  fetch_block8(pool, tiles[ti], r0, k0, x);
+  for (int64_t item = b; item < e; ++item) {
+    const int64_t cti = ti;
+    const int cr0 = r0, ck0 = k0;
+    double v[8];
+    stage_block8(tiles[cti], stg, x, v);
+    if (item + 1 < e) {
+      while (prefix[ti + 1] <= item + 1) ++ti;
+      block_of(tiles[ti], item + 1 - prefix[ti], r0, k0);
+      fetch_block8(pool, tiles[ti], r0, k0, x);
+    }
+    body(tiles[cti], cti, cr0, ck0, v);
+  }
 }
 
 // max |a| per tile; positive doubles order like their bits
@@ -177,15 +210,8 @@ __global__ __launch_bounds__(256) void split_max_kernel(
     unsigned long long* __restrict__ maxbits) {
   __shared__ double stg[kSplitBK][kSplitBR + 1];
   __shared__ double red[256];
-  int64_t b, e, ti;
-  block_range(prefix, ntiles, b, e, ti);
-  for (int64_t item = b; item < e; ++item) {
-    while (prefix[ti + 1] <= item) ++ti;
-    const SplitTile& t = tiles[ti];
-    int r0, k0;
-    block_of(t, item - prefix[ti], r0, k0);
-    double v[8];
-    load_block8(pool, t, r0, k0, stg, v);
+  for_each_block8(pool, tiles, prefix, ntiles, stg,
+                  [&](const SplitTile&, int64_t ti, int, int, const double (&v)[8]) {
     double m = 0.0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) m = fmax(m, fabs(v[i]));
@@ -198,7 +224,7 @@ __global__ __launch_bounds__(256) void split_max_kernel(
     if (threadIdx.x == 0 && red[0] > 0.0)
       atomicMax(&maxbits[ti], (unsigned long long)__double_as_longlong(red[0]));
     __syncthreads();
-  }
+  });
 }
 
 // maxabs (when given) holds each tile's max |a|, else maxbits its bits
@@ -221,17 +247,10 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
     const int64_t* __restrict__ prefix, int64_t ntiles, bf8_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
   __shared__ double stg[kSplitBK][kSplitBR + 1];
-  int64_t b, e, ti;
-  block_range(prefix, ntiles, b, e, ti);
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  for (int64_t item = b; item < e; ++item) {
-    while (prefix[ti + 1] <= item) ++ti;
-    const SplitTile& t = tiles[ti];
-    int r0, k0;
-    block_of(t, item - prefix[ti], r0, k0);
-    double v[8];
-    load_block8(pool, t, r0, k0, stg, v);
-    if (r0 + r >= t.rows_pad) continue;
+  for_each_block8(pool, tiles, prefix, ntiles, stg,
+                  [&](const SplitTile& t, int64_t, int r0, int k0, const double (&v)[8]) {
+    if (r0 + r >= t.rows_pad) return;
     const int sigma = t.sigma;
     bf8_t out[S];
 #pragma unroll
@@ -253,7 +272,7 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
     const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
 #pragma unroll
     for (int s = 0; s < S; ++s) limbs[o + s * plane] = out[s];
-  }
+  });
 }
 
 // INT8 limbs: x = a 2^-sigma with |x| <= 127/128; l_0 = rint(128 x) (|l_0|
@@ -280,17 +299,10 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     const int64_t* __restrict__ prefix, int64_t ntiles, uint64_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
   __shared__ double stg[kSplitBK][kSplitBR + 1];
-  int64_t b, e, ti;
-  block_range(prefix, ntiles, b, e, ti);
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  for (int64_t item = b; item < e; ++item) {
-    while (prefix[ti + 1] <= item) ++ti;
-    const SplitTile& t = tiles[ti];
-    int r0, k0;
-    block_of(t, item - prefix[ti], r0, k0);
-    double v[8];
-    load_block8(pool, t, r0, k0, stg, v);
-    if (r0 + r >= t.rows_pad) continue;
+  for_each_block8(pool, tiles, prefix, ntiles, stg,
+                  [&](const SplitTile& t, int64_t, int r0, int k0, const double (&v)[8]) {
+    if (r0 + r >= t.rows_pad) return;
     const int sigma = t.sigma;
     uint32_t w[S][8];  // limb s of element i in the low byte of w[s][i]
 #pragma unroll
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
 #pragma unroll
     for (int s = 0; s < S; ++s) limbs[o + s * plane] = out[s];
-  }
+  });
 }
 
 template <int S>
