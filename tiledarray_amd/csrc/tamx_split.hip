@@ -44,10 +44,20 @@ namespace {
 using namespace tamx::split;
 constexpr int SBM = 64, SBN = 64;  // macro tile of split_gemm_kernel
 
-// The split passes walk 64-row x 32-k blocks of every tile's limb layout;
-// thread t of a 256-thread workgroup owns row t / 4 and the 8 consecutive k
-// of unit t % 4 (one 16-byte store per limb plane).
-constexpr int kSplitBR = 64, kSplitBK = 32;
+// The split passes walk 16-row x 128-k blocks of every tile's limb layout;
+// thread t of a 256-thread workgroup owns row t / 16 and the 8 consecutive
+// k from 8 (t % 16) (one 8-byte INT8 or 16-byte BF16 store per limb
+// plane), so the 16 threads of a row write whole 128-byte lines of every
+// plane.  k_pad (a multiple of 32 or 64) need not be a multiple of 128: the
+// threads past it load zeros and store nothing.
+constexpr int kSplitBR = 16, kSplitBK = 128, kSplitTQ = kSplitBK / 8;
+// LDS staging: element (row r, k) at r * kSplitRS + k + k / 8; the pad
+// double after every 8 k spreads a row's 16 thread groups over all banks
+constexpr int kSplitRS = kSplitBK + kSplitBK / 8 + 1;
+static_assert(kSplitBR * kSplitBK == 256 * 8 && 256 % kSplitBK == 0 && 256 % kSplitBR == 0,
+              "one 8-element group per thread");
+
+__host__ __device__ inline int split_blocks_k(int k_pad) { return (k_pad + kSplitBK - 1) / kSplitBK; }
 
 __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t item) {
   int64_t lo = 0, hi = n - 1;
@@ -87,12 +97,12 @@ __device__ inline int64_t mixed_radix_offset(int x, int n, const int32_t* ext,
   return off;
 }
 
-// A block of 64 limb rows x 32 k is read with the lanes along the stored
-// rows (a transposed tile, or a general one whose last row mode is the
-// source's stride-1 mode: element j of thread tid is k (tid >> 6) + 4 j, row
-// tid & 63) or along k (element j: k tid & 31, row (tid >> 5) + 8 j), so a
-// wave's global reads are contiguous, and turned into the thread layout
-// (row tid >> 2, k 8 (tid & 3) .. + 8) through LDS.
+// A block is read with the lanes along the stored rows (a transposed tile,
+// or a general one whose last row mode is the source's stride-1 mode:
+// element j of thread tid is row tid % 16, k tid / 16 + 16 j) or along k
+// (element j: k tid % 128, row tid / 128 + 2 j), so a wave's global reads
+// are contiguous, and turned into the thread layout (row tid / 16, k
+// 8 (tid % 16) .. + 8) through LDS.
 __device__ inline bool lanes_along_rows(const SplitTile& t) {
   return t.gen_nr > 0 ? t.gr_str[t.gen_nr - 1] == 1 : t.transpose != 0;
 }
@@ -104,69 +114,73 @@ __device__ inline void fetch_block8(const double* __restrict__ pool, const Split
   const int tid = threadIdx.x;
   if (t.gen_nr > 0) {
     if (lanes_along_rows(t)) {
-      const int gr = r0 + (tid & 63);
+      const int gr = r0 + tid % kSplitBR;
       const bool row_ok = gr < t.out_rows;
       const int64_t base =
           t.src_off + (row_ok ? mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str) : 0);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int gk = k0 + (tid >> 6) + 4 * j;
+        const int gk = k0 + tid / kSplitBR + (256 / kSplitBR) * j;
         x[j] = row_ok && gk < t.out_k
                    ? pool[base + mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str)]
                    : 0.0;
       }
     } else {
-      const int gk = k0 + (tid & 31);
+      const int gk = k0 + tid % kSplitBK;
       const bool k_ok = gk < t.out_k;
       const int64_t base =
           t.src_off + (k_ok ? mixed_radix_offset(gk, t.gen_nk, t.gk_ext, t.gk_str) : 0);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int gr = r0 + (tid >> 5) + 8 * j;
+        const int gr = r0 + tid / kSplitBK + (256 / kSplitBK) * j;
         x[j] = k_ok && gr < t.out_rows
                    ? pool[base + mixed_radix_offset(gr, t.gen_nr, t.gr_ext, t.gr_str)]
                    : 0.0;
       }
     }
   } else if (t.transpose) {
-    const int gr = r0 + (tid & 63);
+    const int gr = r0 + tid % kSplitBR;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int gk = k0 + (tid >> 6) + 4 * j;
+      const int gk = k0 + tid / kSplitBR + (256 / kSplitBR) * j;
       x[j] = gk < t.out_k && gr < t.out_rows ? pool[t.src_off + (int64_t)gk * t.cols + gr] : 0.0;
     }
   } else {
-    const int gk = k0 + (tid & 31);
+    const int gk = k0 + tid % kSplitBK;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int gr = r0 + (tid >> 5) + 8 * j;
+      const int gr = r0 + tid / kSplitBK + (256 / kSplitBK) * j;
       x[j] = gk < t.out_k && gr < t.out_rows ? pool[t.src_off + (int64_t)gr * t.cols + gk] : 0.0;
     }
   }
 }
 
 // The fetched elements through LDS into the thread layout: v[i] is row
-// tid >> 2, k 8 (tid & 3) + i of the block.  Every thread of the block
+// tid / 16, k 8 (tid % 16) + i of the block.  Every thread of the block
 // calls it (two barriers).
-__device__ inline void stage_block8(const SplitTile& t, double (*stg)[kSplitBR + 1],
-                                    const double (&x)[8], double (&v)[8]) {
+__device__ inline int stg_index(int r, int k) { return r * kSplitRS + k + k / 8; }
+
+__device__ inline void stage_block8(const SplitTile& t, double* stg, const double (&x)[8],
+                                    double (&v)[8]) {
   const int tid = threadIdx.x;
   if (lanes_along_rows(t)) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) stg[(tid >> 6) + 4 * j][tid & 63] = x[j];
+    for (int j = 0; j < 8; ++j)
+      stg[stg_index(tid % kSplitBR, tid / kSplitBR + (256 / kSplitBR) * j)] = x[j];
   } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) stg[tid & 31][(tid >> 5) + 8 * j] = x[j];
+    for (int j = 0; j < 8; ++j)
+      stg[stg_index(tid / kSplitBK + (256 / kSplitBK) * j, tid % kSplitBK)] = x[j];
   }
   __syncthreads();
-  const int r = tid >> 2, q = tid & 3;
+  const int r = tid / kSplitTQ, q = tid % kSplitTQ;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = stg[8 * q + i][r];
+  for (int i = 0; i < 8; ++i) v[i] = stg[stg_index(r, 8 * q + i)];
   __syncthreads();
 }
 
 __device__ inline void block_of(const SplitTile& t, int64_t local, int& r0, int& k0) {
-  const int nbk = t.k_pad / kSplitBK;
+  const int nbk = split_blocks_k(t.k_pad);
   r0 = (int)(local / nbk) * kSplitBR;
   k0 = (int)(local % nbk) * kSplitBK;
 }
@@ -180,7 +194,7 @@ template <class Body>
 __device__ inline void for_each_block8(const double* __restrict__ pool,
                                        const SplitTile* __restrict__ tiles,
                                        const int64_t* __restrict__ prefix, int64_t ntiles,
-                                       double (*stg)[kSplitBR + 1], Body&& body) {
+                                       double* stg, Body&& body) {
   int64_t b, e, ti;
   block_range(prefix, ntiles, b, e, ti);
   if (b >= e) return;
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(256) void split_max_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
     const int64_t* __restrict__ prefix, int64_t ntiles,
     unsigned long long* __restrict__ maxbits) {
-  __shared__ double stg[kSplitBK][kSplitBR + 1];
+  __shared__ double stg[kSplitBR * kSplitRS];
   __shared__ double red[256];
   for_each_block8(pool, tiles, prefix, ntiles, stg,
                   [&](const SplitTile&, int64_t ti, int, int, const double (&v)[8]) {
@@ -246,11 +260,11 @@ __global__ __launch_bounds__(256) void split_limbs_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
     const int64_t* __restrict__ prefix, int64_t ntiles, bf8_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
-  __shared__ double stg[kSplitBK][kSplitBR + 1];
-  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  __shared__ double stg[kSplitBR * kSplitRS];
+  const int r = threadIdx.x / kSplitTQ, q = threadIdx.x % kSplitTQ;
   for_each_block8(pool, tiles, prefix, ntiles, stg,
                   [&](const SplitTile& t, int64_t, int r0, int k0, const double (&v)[8]) {
-    if (r0 + r >= t.rows_pad) return;
+    if (r0 + r >= t.rows_pad || k0 + 8 * q >= t.k_pad) return;
     const int sigma = t.sigma;
     bf8_t out[S];
 #pragma unroll
@@ -298,11 +312,11 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,
     const int64_t* __restrict__ prefix, int64_t ntiles, uint64_t* __restrict__ limbs) {
 #pragma clang fp contract(off)
-  __shared__ double stg[kSplitBK][kSplitBR + 1];
-  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  __shared__ double stg[kSplitBR * kSplitRS];
+  const int r = threadIdx.x / kSplitTQ, q = threadIdx.x % kSplitTQ;
   for_each_block8(pool, tiles, prefix, ntiles, stg,
                   [&](const SplitTile& t, int64_t, int r0, int k0, const double (&v)[8]) {
-    if (r0 + r >= t.rows_pad) return;
+    if (r0 + r >= t.rows_pad || k0 + 8 * q >= t.k_pad) return;
     const int sigma = t.sigma;
     uint32_t w[S][8];  // limb s of element i in the low byte of w[s][i]
 #pragma unroll
@@ -1050,7 +1064,7 @@ int tamx_split_plan_create_typed(tamx_context* ctx, int limb_type, int nslices, 
     st.limb_off = off;
     off += (int64_t)nslices * st.rows_pad * st.k_pad;
     max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
-    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * (st.k_pad / kSplitBK);
+    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * split_blocks_k(st.k_pad);
   }
   prefix[ntiles] = blocks;
   auto* plan = new (std::nothrow) tamx_split_plan();
@@ -1146,7 +1160,7 @@ int tamx_split_plan_create_general_typed(tamx_context* ctx, int limb_type, int n
     st.limb_off = off;
     off += (int64_t)nslices * st.rows_pad * st.k_pad;
     max_kpad = std::max<int64_t>(max_kpad, st.k_pad);
-    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * (st.k_pad / kSplitBK);
+    blocks += (int64_t)((st.rows_pad + kSplitBR - 1) / kSplitBR) * split_blocks_k(st.k_pad);
   }
   prefix[ntiles] = blocks;
   auto* plan = new (std::nothrow) tamx_split_plan();
