@@ -209,8 +209,9 @@ int tamx_shape_scale(tamx_context* ctx, float* norms, int rank,
  * order (K x M if op_a), right K x N (N x K if op_b), k_sizes the K volumes
  * of the contracted tiles.  est = abs_factor * sum_k l*r computed in float,
  * k ascending, products and sums rounded separately; est < threshold -> 0.
- * nbatch independent problems may be stacked (batch strides M*K, K*N, K,
- * M*N).  zero_count as above. */
+ * K = 0 is the outer product est = abs_factor * l_i * r_j.  nbatch
+ * independent problems may be stacked (batch strides M*K, K*N, K, M*N; M
+ * and N for the operands of an outer product).  zero_count as above. */
 int tamx_shape_gemm_f32(tamx_context* ctx, const float* left,
                         const float* right, int op_a, int op_b, int64_t M,
                         int64_t N, int64_t K, int64_t nbatch,

@@ -177,6 +177,37 @@ def test_blocksparse_screening_bit_exact_and_values(ctx):
     assert err <= FP64_TOL, err
 
 
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+@pytest.mark.parametrize("M,N,K", [(13, 5, 9), (4, 11, 3), (1, 1, 1), (7, 6, 0)])
+def test_shape_gemm_direct_bit_exact(ctx, op_a, op_b, M, N, K):
+    # the shape estimate through the C ABI, two problems in one batch, against
+    # the oracle: estimate bit for bit and the count of screened-out tiles
+    import ctypes
+    rng = np.random.default_rng(1000 + 17 * M + 5 * N + K + 2 * op_a + op_b)
+    nb = 2
+    kk = max(K, 1)
+    left = (rng.random((nb, M * kk)) * 2.0 ** rng.integers(-3, 2, (nb, M * kk))).astype(np.float32)
+    right = (rng.random((nb, kk * N)) * 2.0 ** rng.integers(-3, 2, (nb, kk * N))).astype(np.float32)
+    ksz = rng.integers(1, 300, (nb, kk)).astype(np.float32)
+    # a threshold that screens out about half of the first problem's tiles
+    est0, _ = oracle.shape_gemm(left[0], right[0], op_a, op_b, M, N, K, ksz[0], 1.5, 0.0)
+    thr = np.float32(np.median(est0))
+    dl, dr, dk = (torch.as_tensor(x.reshape(-1), device="cuda") for x in (left, right, ksz))
+    out = torch.empty(nb * M * N, dtype=torch.float32, device="cuda")
+    zc = torch.zeros(1, dtype=torch.int64, device="cuda")
+    E.call("tamx_shape_gemm_f32", ctx.ptr, dl.data_ptr(), dr.data_ptr(), op_a, op_b, M, N, K, nb,
+           dk.data_ptr(), ctypes.c_float(1.5), ctypes.c_float(thr), out.data_ptr(), zc.data_ptr(),
+           ctx.stream())
+    torch.cuda.synchronize()
+    zsum = 0
+    for b in range(nb):
+        est, z = oracle.shape_gemm(left[b], right[b], op_a, op_b, M, N, K, ksz[b], 1.5, thr)
+        np.testing.assert_array_equal(out.cpu().numpy()[b * M * N:(b + 1) * M * N], est.reshape(-1))
+        zsum += z
+    assert 0 < zsum < nb * M * N or M * N == 1, zsum
+    assert int(zc.item()) == zsum
+
+
 def test_blocksparse_integer_bit_exact(ctx):
     A, B, _ = W.blocksparse_gemm(n=2000, wlo=30, whi=90, seed=12, integer=True)
     E.make_sparse(ctx, A, 1e-8)

@@ -228,27 +228,48 @@ __global__ void shape_gemm_kernel(ShapeGemmParams p) {
   const int64_t b = g / (p.M * p.N);
   const int64_t ij = g % (p.M * p.N);
   const int64_t i = ij / p.N, j = ij % p.N;
-  const float* L = p.left + b * p.M * p.K;
-  const float* R = p.right + b * p.K * p.N;
+  const int64_t kk = p.K > 0 ? p.K : 1;  // an outer product's operands hold M and N values
+  const float* L = p.left + b * p.M * kk;
+  const float* R = p.right + b * kk * p.N;
   const float* ks = p.ksz + b * p.K;
   float v;
   if (p.K == 0) {
     v = (L[i] * R[j]) * p.abs_factor;  // outer product (:1666-1678)
   } else {
+    // storage indices sl of (i, k) in left and sr of (k, j) in right, with
+    // sl % K and (sr / N, sr % N) carried along as k steps sl by dl and sr
+    // by dr (no division in the loop)
+    const int64_t K = p.K, N = p.N;
+    const int64_t dl = p.op_a ? p.M : 1, dr = p.op_b ? 1 : N;
+    int64_t sl = p.op_a ? i : i * K, sr = p.op_b ? j * K : j;
+    int64_t lk = sl % K, rq = sr / N, rr = sr % N;
+    const int64_t dlk = dl % K, drq = dr / N, drr = dr % N;
     float acc = 0.0f;
-    for (int64_t k = 0; k < p.K; ++k) {
-      const int64_t sl = p.op_a ? k * p.M + i : i * p.K + k;
-      const int64_t sr = p.op_b ? j * p.K + k : k * p.N + j;
-      const float l = L[sl] * ks[sl % p.K];
-      const float r = R[sr] * ks[sr / p.N];
+    for (int64_t k = 0; k < K; ++k) {
+      const float l = L[sl] * ks[lk];
+      const float r = R[sr] * ks[rq];
       const float prod = l * r;
       acc = acc + prod;
+      sl += dl;
+      lk += dlk;
+      if (lk >= K) lk -= K;
+      sr += dr;
+      rq += drq;
+      rr += drr;
+      if (rr >= N) {
+        rr -= N;
+        ++rq;
+      }
     }
     v = p.abs_factor * acc;
   }
-  if (v < p.threshold) {
-    v = 0.0f;
-    if (p.zero_count) atomicAdd((unsigned long long*)p.zero_count, 1ull);
+  const bool zero = v < p.threshold;
+  if (zero) v = 0.0f;
+  if (p.zero_count) {  // one atomic per wave
+    const unsigned long long zmask = __ballot(zero);
+    const unsigned long long amask = __ballot(1);
+    if (zmask && __lane_id() == (unsigned)(__ffsll((long long)amask) - 1))
+      atomicAdd((unsigned long long*)p.zero_count, (unsigned long long)__popcll(zmask));
   }
   p.result[g] = v;
 }
