@@ -160,6 +160,14 @@ def test_blocksparse_screening_bit_exact_and_values(ctx):
     sb, zb = oracle.scale_tile_norms(ob.reshape(B.trange.tiles_shape), ext, thr)
     np.testing.assert_array_equal(A.shape_norms.cpu().numpy(), sa.reshape(-1))
     np.testing.assert_array_equal(B.shape_norms.cpu().numpy(), sb.reshape(-1))
+    # the scaling on its own, with the count of screened-out tiles
+    for o, s_ref, z_ref in ((oa, sa, za), (ob, sb, zb)):
+        n = torch.as_tensor(o, device="cuda").clone()
+        zc = torch.zeros(1, dtype=torch.int64, device="cuda")
+        E.scale_norms(ctx, n, A.trange, thr, zero_count=zc)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(n.cpu().numpy(), s_ref.reshape(-1))
+        assert 0 < z_ref < n.numel() and int(zc.item()) == z_ref
     c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr)
     c.run()
     torch.cuda.synchronize()

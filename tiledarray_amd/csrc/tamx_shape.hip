@@ -148,6 +148,15 @@ int tile_norms(tamx_context* ctx, const tamx_norm_plan* plan, const T* pool,
 
 // ------------------------------------------------------- shape scaling --
 
+// Adds the number of the wave's active lanes with `zero` set to *count: one
+// atomic per wave (the lanes that returned early take no part).
+__device__ inline void count_zeros(bool zero, int64_t* count) {
+  const unsigned long long zmask = __ballot(zero);
+  const unsigned long long amask = __ballot(1);
+  if (zmask && __lane_id() == (unsigned)(__ffsll((long long)amask) - 1))
+    atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(zmask));
+}
+
 constexpr int kMaxRank = 8;
 
 struct ScaleParams {
@@ -197,10 +206,9 @@ __global__ void shape_scale_kernel(ScaleParams p) {
     const float y = inv_rop(p, sizes, idx, middle, p.rank);
     v = v * (x * y);  // norm *= x * y (:205-213)
   }
-  if (v < p.threshold) {
-    v = 0.0f;
-    if (p.zero_count) atomicAdd((unsigned long long*)p.zero_count, 1ull);
-  }
+  const bool zero = v < p.threshold;
+  if (zero) v = 0.0f;
+  if (p.zero_count) count_zeros(zero, p.zero_count);
   p.norms[g] = v;
 }
 
@@ -265,12 +273,7 @@ __global__ void shape_gemm_kernel(ShapeGemmParams p) {
   }
   const bool zero = v < p.threshold;
   if (zero) v = 0.0f;
-  if (p.zero_count) {  // one atomic per wave
-    const unsigned long long zmask = __ballot(zero);
-    const unsigned long long amask = __ballot(1);
-    if (zmask && __lane_id() == (unsigned)(__ffsll((long long)amask) - 1))
-      atomicAdd((unsigned long long*)p.zero_count, (unsigned long long)__popcll(zmask));
-  }
+  if (p.zero_count) count_zeros(zero, p.zero_count);
   p.result[g] = v;
 }
 
