@@ -253,6 +253,9 @@ __global__ void shape_gemm_kernel(ShapeGemmParams p) {
     int64_t lk = sl % K, rq = sr / N, rr = sr % N;
     const int64_t dlk = dl % K, drq = dr / N, drr = dr % N;
     float acc = 0.0f;
+    // unrolled so that several steps' loads are in flight; the sum still
+    // runs k ascending
+#pragma unroll 8
     for (int64_t k = 0; k < K; ++k) {
       const float l = L[sl] * ks[lk];
       const float r = R[sr] * ks[rq];
