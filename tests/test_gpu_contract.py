@@ -216,6 +216,42 @@ def test_shape_gemm_direct_bit_exact(ctx, op_a, op_b, M, N, K):
     assert int(zc.item()) == zsum
 
 
+def test_pair_filter_matches_oracle(ctx):
+    # the per-pair rule (|A_ik| |B_kj| < threshold / Kt drops the pair, the
+    # rule TA compiles out) against the oracle: the same pairs, values within
+    # the FP64 tolerance; the threshold drops about half of the pairs
+    thr = 1e-8
+    A, B, _ = W.blocksparse_gemm(n=2000, wlo=30, whi=90, seed=14, integer=True)
+    E.make_sparse(ctx, A, thr)
+    E.make_sparse(ctx, B, thr)
+    sa, sb = A.shape_norms.cpu().numpy(), B.shape_norms.cpu().numpy()
+    a_present = (A.offsets >= 0) & (sa >= thr)
+    b_present = (B.offsets >= 0) & (sb >= thr)
+    nt = A.trange.tiles_shape[0]
+    pa, pb = a_present.reshape(nt, nt), b_present.reshape(nt, nt)
+    na, nb = sa.reshape(nt, nt), sb.reshape(nt, nt)
+    prods = [na[i, k] * nb[k, j] for i in range(nt) for k in range(nt) for j in range(nt)
+             if pa[i, k] and pb[k, j]]
+    pthr = float(np.float32(np.median(prods)) * np.float32(nt))
+    c0 = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr)
+    c0.run()
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, pair_threshold=pthr)
+    c.run()
+    torch.cuda.synchronize()
+    a_off, b_off = A.offsets.copy(), B.offsets.copy()
+    a_off[~a_present] = -1
+    b_off[~b_present] = -1
+    c_off = c.c_off.cpu().numpy()
+    Cref, npairs = oracle.contract(c.op_a, c.op_b, c.mext_h, c.next_h, c.kext_h, host_pool(A),
+                                   a_off, host_pool(B), b_off, c_off, c.c_capacity,
+                                   alpha=c.alpha, a_norms=sa, b_norms=sb,
+                                   pair_filter_threshold=pthr)
+    n0 = c0.counters()["npairs"]
+    assert c.counters()["npairs"] == npairs and 0 < npairs < n0
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
 def test_blocksparse_integer_bit_exact(ctx):
     A, B, _ = W.blocksparse_gemm(n=2000, wlo=30, whi=90, seed=12, integer=True)
     E.make_sparse(ctx, A, 1e-8)

@@ -311,19 +311,28 @@ struct WlParams {
   int64_t task_capacity, pair_capacity;
 };
 
-__device__ inline bool a_present(const tamx_contraction_desc& d, int64_t ai) {
-  return d.a_off[ai] >= 0 && (!d.a_norms || !(d.a_norms[ai] < d.a_threshold));
-}
 __device__ inline bool b_present(const tamx_contraction_desc& d, int64_t bi) {
   return d.b_off[bi] >= 0 && (!d.b_norms || !(d.b_norms[bi] < d.b_threshold));
 }
-__device__ inline bool pair_kept(const tamx_contraction_desc& d, int64_t ai,
-                                 int64_t bi, float thr_k) {
+// The A side of a pair (offset and norm of A_ik), read one k ahead of its
+// use by the work-list loops so that the next load is in flight while the
+// current k is tested (most A_ik are absent in a banded pattern, and the
+// test of an absent one waits only on this load).
+struct ASide {
+  int64_t off;
+  float norm;
+};
+__device__ inline ASide a_side(const tamx_contraction_desc& d, int64_t ai) {
+  return {d.a_off[ai], d.a_norms ? d.a_norms[ai] : 0.0f};
+}
+// Whether (A_ik, B_kj) contributes: both stored and not screened out, and
+// (optional per-pair rule) |A_ik| |B_kj| >= thr_k.
+__device__ inline bool pair_kept_a(const tamx_contraction_desc& d, const ASide& a, int64_t bi,
+                                   float thr_k) {
 #pragma clang fp contract(off)
-  if (!a_present(d, ai) || !b_present(d, bi)) return false;
-  if (thr_k > 0.0f && d.a_norms && d.b_norms &&
-      d.a_norms[ai] * d.b_norms[bi] < thr_k)
-    return false;
+  if (!(a.off >= 0 && (!d.a_norms || !(a.norm < d.a_threshold)))) return false;
+  if (!b_present(d, bi)) return false;
+  if (thr_k > 0.0f && d.a_norms && d.b_norms && a.norm * d.b_norms[bi] < thr_k) return false;
   return true;
 }
 
@@ -350,10 +359,15 @@ __global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
       const float thr_k = d.pair_threshold > 0.0f
                               ? d.pair_threshold / (float)d.Kt
                               : 0.0f;
+      auto ai_of = [&](int64_t k) {
+        return b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+      };
+      ASide next = d.Kt > 0 ? a_side(d, ai_of(0)) : ASide{-1, 0.0f};
       for (int64_t k = 0; k < d.Kt; ++k) {
-        const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+        const ASide a = next;
+        if (k + 1 < d.Kt) next = a_side(d, ai_of(k + 1));
         const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
-        if (pair_kept(d, ai, bi, thr_k)) {
+        if (pair_kept_a(d, a, bi, thr_k)) {
           ++np;
           flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k];
         }
@@ -441,13 +455,19 @@ __global__ __launch_bounds__(kWlThreads) void wl_fill_kernel(WlParams p) {
   const float thr_k =
       d.pair_threshold > 0.0f ? d.pair_threshold / (float)d.Kt : 0.0f;
   int32_t q = pb;
+  auto ai_of = [&](int64_t k) {
+    return b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+  };
+  ASide next = d.Kt > 0 ? a_side(d, ai_of(0)) : ASide{-1, 0.0f};
   for (int64_t k = 0; k < d.Kt; ++k) {
-    const int64_t ai = b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
+    const ASide a = next;
+    if (k + 1 < d.Kt) next = a_side(d, ai_of(k + 1));
+    const int64_t ai = ai_of(k);
     const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
-    if (!pair_kept(d, ai, bi, thr_k)) continue;
+    if (!pair_kept_a(d, a, bi, thr_k)) continue;
     const int32_t kk = d.kext[b * d.Kt + k];
     tamx_gemm_pair pr;
-    pr.a_off = d.a_off[ai];
+    pr.a_off = a.off;
     pr.b_off = d.b_off[bi];
     pr.k = kk;
     pr.lda = d.op_a ? m : kk;
