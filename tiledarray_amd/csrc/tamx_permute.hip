@@ -21,7 +21,6 @@ namespace {
 constexpr int kMaxRank = 8;
 constexpr int kTS = 32;       // transpose tile edge
 constexpr int kCopyRun = 2048;  // elements per copy item
-constexpr int kCopyItem = 8192;  // elements per item of a copy of short runs
 // a transposed plane of at most this many elements moves through LDS whole
 // (one item per plane, no partial edge tiles)
 constexpr int kPlaneMax = 96 * 96;
@@ -38,10 +37,6 @@ struct PermTile {
   int32_t s_dim;  // fastest source mode (q - 1)
   int32_t t_dim;  // source mode that is fastest in the result
   int32_t nbs, nbt;  // blocks along s and t (0 for a whole-plane tile)
-  // contiguous-run copies of runs shorter than kCopyRun: an item copies
-  // `rpi` runs along mode u_dim (the result's next-fastest mode), -1: one
-  // run chunk per item
-  int32_t u_dim, rpi;
 };
 
 }  // namespace
@@ -72,41 +67,6 @@ template <typename T>
 __device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ dst,
                                     const PermTile& t, int local, T* plane) {
   const int q = t.q, sd = t.s_dim, td = t.t_dim;
-
-  if (sd == td && t.u_dim >= 0) {
-    // contiguous-run copy of short runs: item = (other modes, block of rpi
-    // runs along u); thread tid copies element tid % len of runs
-    // tid / len, + rows per pass, ... (len <= 512), else strides along the run
-    const int len = t.ext[sd], ud = t.u_dim;
-    const int nb = (t.ext[ud] + t.rpi - 1) / t.rpi;
-    const int ub = local % nb;
-    local /= nb;
-    int64_t so = t.src_off, doff = t.dst_off;
-    for (int d = q - 1; d >= 0; --d) {
-      if (d == sd || d == ud) continue;
-      const int x = local % t.ext[d];
-      local /= t.ext[d];
-      so += x * t.sstride[d];
-      doff += x * t.dstride[d];
-    }
-    const int r0 = ub * t.rpi, nr = min(t.rpi, t.ext[ud] - r0);
-    so += (int64_t)r0 * t.sstride[ud];
-    doff += (int64_t)r0 * t.dstride[ud];
-    const int64_t us = t.sstride[ud], ud_ = t.dstride[ud];
-    if (len <= (int)blockDim.x) {
-      const int rpp = (int)blockDim.x / len;
-      const int row = (int)threadIdx.x / len, x = (int)threadIdx.x - row * len;
-      if (row < rpp) {
-#pragma unroll 4
-        for (int r = row; r < nr; r += rpp) dst[doff + r * ud_ + x] = src[so + r * us + x];
-      }
-    } else {
-      for (int r = 0; r < nr; ++r)
-        for (int x = threadIdx.x; x < len; x += blockDim.x)
-          dst[doff + r * ud_ + x] = src[so + r * us + x];
-    }
-    return;
-  }
 
   if (sd == td) {
     // contiguous-run copy: item = (other modes, run block)
@@ -315,22 +275,8 @@ int tamx_permute_plan_create(tamx_context* ctx, int rank, const int32_t* perm,
     for (int d = 0; d < q; ++d)
       if (t.dstride[d] == 1) t.t_dim = d;
     int64_t nitems = 0;
-    t.u_dim = -1;
-    t.rpi = 0;
     if (vol == 0) {
       nitems = 0;
-    } else if (t.t_dim == t.s_dim && q >= 2 && t.ext[t.s_dim] < kCopyRun) {
-      // short runs: rpi runs along u (the other mode of smallest result
-      // stride, so the rows an item writes are adjacent) per item, about
-      // kCopyItem elements
-      int u = -1;
-      for (int d = 0; d < q; ++d)
-        if (d != t.s_dim && (u < 0 || t.dstride[d] < t.dstride[u])) u = d;
-      t.u_dim = u;
-      t.rpi = (int32_t)std::min<int64_t>(t.ext[u], std::max<int64_t>(1, kCopyItem / t.ext[t.s_dim]));
-      nitems = (t.ext[u] + t.rpi - 1) / t.rpi;
-      for (int d = 0; d < q; ++d)
-        if (d != t.s_dim && d != u) nitems *= t.ext[d];
     } else if (t.t_dim == t.s_dim) {
       nitems = (t.ext[t.s_dim] + kCopyRun - 1) / kCopyRun;
       for (int d = 0; d < q; ++d)
