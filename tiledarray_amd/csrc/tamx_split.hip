@@ -68,7 +68,7 @@ __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t it
   return lo;
 }
 
-// The contiguous range [b, e) of 64 x 32 blocks of this workgroup and the
+// The contiguous range [b, e) of 16 x 128 blocks of this workgroup and the
 // tile of its first block: the tile table is searched once per workgroup,
 // then followed forward (tile t covers blocks [prefix[t], prefix[t+1])).
 __device__ inline void block_range(const int64_t* prefix, int64_t ntiles, int64_t& b,
@@ -254,7 +254,7 @@ __global__ void split_sigma_kernel(SplitTile* tiles, const unsigned long long* m
   tiles[t].sigma = sigma;
 }
 
-// Writes the S limb planes of every 64 x 32 block of the limb layout.
+// Writes the S limb planes of every 16 x 128 block of the limb layout.
 template <int S>
 __global__ __launch_bounds__(256) void split_limbs_kernel(
     const double* __restrict__ pool, const SplitTile* __restrict__ tiles,

@@ -96,7 +96,7 @@ struct tamx_split_plan {
   int32_t nslices = 0;
   int32_t limb_type = 0;    // TAMX_LIMB_BF16 or TAMX_LIMB_I8
   int64_t limb_elems = 0;   // size of the limb pool in limbs (bf16 or int8)
-  int64_t nblocks = 0;      // 64 x 32 blocks of the limb layouts over all tiles
+  int64_t nblocks = 0;      // 16 x 128 blocks of the limb layouts over all tiles
   tamx::split::SplitTile* tiles = nullptr;  // device
   int64_t* block_prefix = nullptr;          // device, ntiles + 1
   unsigned long long* maxbits = nullptr;    // device, per-tile max |a| bits
