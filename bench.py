@@ -203,9 +203,11 @@ def main():
             if sparse:
                 screening(bool(c.nslices))
             if c.nslices:
-                c.split_step(maxabs=(maxabs[id(A)], maxabs[id(B)]) if sparse else None)
-            c.shape_step()
-            c.worklist_step()
+                # the limb split overlapped with the shape estimate and work list
+                c.split_and_plan(maxabs=(maxabs[id(A)], maxabs[id(B)]) if sparse else None)
+            else:
+                c.shape_step()
+                c.worklist_step()
             if events is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
@@ -362,9 +364,10 @@ def bench_ladder(args):
         perm_norms.copy_(V.shape_norms.view(V.trange.tiles_shape).permute(*inv).reshape(-1))
         if c.nslices:
             perm_max.copy_(maxabs[id(V)].view(V.trange.tiles_shape).permute(*inv).reshape(-1))
-            c.split_step(maxabs=(maxabs[id(T)], perm_max))
-        c.shape_step()
-        c.worklist_step()
+            c.split_and_plan(maxabs=(maxabs[id(T)], perm_max))
+        else:
+            c.shape_step()
+            c.worklist_step()
         if events is not None:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)

@@ -452,6 +452,7 @@ class Contraction:
         grid = self.Mt * self.Nt
         self.wl = WorkList(ctx, grid, max(1, grid * self.Kt), grid)
         self.pair_threshold = float(pair_threshold)
+        self._side = None  # side stream of split_and_plan, made on first use
         self._desc = self._make_desc()
         self.nslices = 0
         self.limb_type = _lib.LIMB_I8 if precision.startswith("i8x") else _lib.LIMB_BF16
@@ -578,6 +579,28 @@ class Contraction:
                 call("tamx_split_f64_premax", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
                      m.data_ptr(), sp.limbs.data_ptr(), self.ctx.stream())
 
+    def split_and_plan(self, maxabs=None):
+        """The limb split of both operands and the planning steps (shape
+        estimate, work list), which do not depend on each other: the split
+        runs on a side stream, the planning on the current one, and the
+        current stream waits for both before it returns, so a gemm_step
+        enqueued after it sees the limbs."""
+        dev = self.ctx.device
+        main = torch.cuda.current_stream(dev)
+        if self._side is None:
+            self._side = torch.cuda.Stream(dev)  # default priority
+        side = self._side
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            self.split_step(maxabs)
+        if maxabs is not None:
+            for m in maxabs:
+                if m is not None:
+                    m.record_stream(side)
+        self.shape_step()
+        self.worklist_step()
+        main.wait_stream(side)
+
     def sigmas(self):
         out = []
         for sp in (self.a_split, self.b_split):
@@ -686,9 +709,10 @@ class Contraction:
         if self.permute_operands:
             self.permute_step()
         if self.nslices:
-            self.split_step()
-        self.shape_step()
-        self.worklist_step()
+            self.split_and_plan()
+        else:
+            self.shape_step()
+            self.worklist_step()
         self.gemm_step()
 
     # -- results ------------------------------------------------------------
