@@ -719,6 +719,44 @@ def test_split_i8_limbs_closed_form(ctx, nslices, is_right, op):
     assert mx < 2.0 ** sigma and (mx >= 2.0 ** (sigma - 1) or mx * 2.0 ** (8 - sigma) > 127)
 
 
+@pytest.mark.parametrize("e", [-1070, -700, 700, 1000])
+def test_split_i8_limbs_extreme_scales(ctx, e):
+    # tiles from the subnormal range to near the top of the FP64 range: the
+    # scale exponent follows the tile and the limbs equal their definition
+    # bit for bit (x_0 = a 2^(7-sigma) is formed exactly by ldexp)
+    rng = np.random.default_rng(1200 + e)
+    a = np.ldexp(rng.uniform(-1, 1, (23, 70)), e)
+    assert np.all(np.isfinite(a)) and np.abs(a).max() > 0
+    L, sigma, logical = _i8_split_tile(ctx, a, 4, 0, 0)
+    m, k = logical.shape
+    assert np.array_equal(L[:, :m, :k], _i8_limbs_by_definition(logical, sigma, 4))
+    mx = np.abs(a).max()
+    assert mx < np.ldexp(1.0, sigma) and mx >= np.ldexp(1.0, sigma - 2)
+
+
+@pytest.mark.parametrize("precision", ["i8x3", "bf16x3"])
+def test_split_power_of_two_scaled_operands_exact(ctx, precision):
+    # A scaled by 2^600 and B by 2^-600: the per-tile scale exponents move
+    # by 600 each, the limbs stay the same, and the product of integer-valued
+    # tiles equals the oracle's unscaled product bit for bit
+    rng = np.random.default_rng(1300)
+    trm = TiledRange1([0, 40, 130, 200])
+    trk = TiledRange1([0, 17, 64, 150, 151, 300])
+    trn = TiledRange1([0, 96, 97, 180])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, 0, 0, torch.float64, True, fill=0.8)
+    Ah, Bh = A.pool.clone(), B.pool.clone()
+    A.pool.mul_(2.0 ** 600)
+    B.pool.mul_(2.0 ** -600)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision=precision)
+    c.run()
+    torch.cuda.synchronize()
+    A.pool.copy_(Ah)
+    B.pool.copy_(Bh)
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
 @pytest.mark.parametrize("nslices", [2, 3, 5, 6])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
 def test_split_i8_integer_exact(ctx, nslices, op_a, op_b):
