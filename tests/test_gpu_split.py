@@ -719,6 +719,43 @@ def test_split_i8_limbs_closed_form(ctx, nslices, is_right, op):
     assert mx < 2.0 ** sigma and (mx >= 2.0 ** (sigma - 1) or mx * 2.0 ** (8 - sigma) > 127)
 
 
+@pytest.mark.parametrize("precision", ["f64", "bf16x3", "i8x4"])
+@pytest.mark.parametrize("empty", ["A", "B", "both"])
+@pytest.mark.parametrize("sparse", [True, False])
+def test_contraction_with_empty_operand(ctx, precision, empty, sparse):
+    # an operand that stores no tile.  Sparse (screened shapes): the result
+    # shape is zero, so no task, no pair, every result tile absent.  Dense
+    # (no screening): every result tile is a task with no pair, written as
+    # zeros (beta = 0, ContractReduce with nothing to reduce)
+    rng = np.random.default_rng(1400)
+    trm = TiledRange1([0, 40, 130])
+    trk = TiledRange1([0, 17, 64, 150])
+    trn = TiledRange1([0, 96, 97])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, 0, 0, torch.float64, True, fill=0.8)
+    if empty in ("A", "both"):
+        A = DistArray.allocate(TiledRange([trm, trk]), np.zeros((2, 3), bool), dtype=torch.float64)
+    if empty in ("B", "both"):
+        B = DistArray.allocate(TiledRange([trk, trn]), np.zeros((3, 2), bool), dtype=torch.float64)
+    kw = {}
+    if sparse:
+        E.make_sparse(ctx, A, 1e-8)
+        E.make_sparse(ctx, B, 1e-8)
+        kw["threshold"] = 1e-8
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision=precision, **kw)
+    c.C.fill_(7.0)
+    c.run()
+    torch.cuda.synchronize()
+    cnt = c.counters()
+    assert cnt["npairs"] == 0 and cnt["flops"] == 0
+    c_off = c.c_off.cpu().numpy()
+    if sparse:
+        assert cnt["ntasks"] == 0 and (c_off < 0).all()
+    else:
+        assert cnt["ntasks"] == 4 and (c_off >= 0).all()
+        mask = c_off_mask(c_off, c)
+        assert not c.C.cpu().numpy()[mask].any()
+
+
 @pytest.mark.parametrize("e", [-1070, -700, 700, 1000])
 def test_split_i8_limbs_extreme_scales(ctx, e):
     # tiles from the subnormal range to near the top of the FP64 range: the
