@@ -126,3 +126,36 @@ def test_tot_empty_instances(ctx):
     cnt = c.counters()
     assert cnt["ntasks"] == 0 and cnt["npairs"] == 0
     assert (c.c_off.cpu().numpy() < 0).all()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_mode_product_integer_exact(ctx, dtype):
+    # C("n,i,j") = 2 A("n,i,k") * B("n,k,j") with n a Hadamard mode (TA's
+    # batched Summa): integer tiles, some absent, ragged tilings of every
+    # mode; the result equals the per-element product (numpy, exact on
+    # integers) everywhere, zero where no pair contributes
+    from tiledarray_amd import DistArray, TiledRange, TiledRange1
+    rng = np.random.default_rng(1500)
+    trn = TiledRange1([0, 3, 7, 8])
+    tri = TiledRange1([0, 20, 45])
+    trk = TiledRange1([0, 17, 40, 41])
+    trj = TiledRange1([0, 33, 50])
+    ma = rng.random((3, 2, 3)) < 0.7
+    mb = rng.random((3, 3, 2)) < 0.7
+    A = DistArray.allocate(TiledRange([trn, tri, trk]), ma, dtype=dtype)
+    B = DistArray.allocate(TiledRange([trn, trk, trj]), mb, dtype=dtype)
+    for X in (A, B):
+        X.pool.copy_(torch.as_tensor(rng.integers(-9, 10, X.pool.numel()), dtype=dtype))
+    c = E.FusedContraction(ctx, A, B, alpha=2.0)
+    c.run()
+    R = c.result()
+    torch.cuda.synchronize()
+    ref = 2.0 * np.einsum("nik,nkj->nij", A.to_dense().astype(np.float64),
+                          B.to_dense().astype(np.float64))
+    got = R.to_dense().astype(np.float64)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got, ref)
+    # pairs: per element, the k tiles where both slices are stored
+    t = np.repeat(np.arange(3), trn.extents)
+    expect = int(sum((ma[t[n]][:, :, None] & mb[t[n]][None, :, :]).sum() for n in range(8)))
+    assert c.counters()["npairs"] == expect

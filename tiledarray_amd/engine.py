@@ -892,3 +892,84 @@ class BatchedContraction:
         torch.cuda.current_stream(self.ctx.device).wait_stream(side)
         torch.cuda.synchronize()
         return Graph(self.ctx, self.run)
+
+
+class FusedContraction:
+    """C("n,i,j") = alpha * A("n,i,k") * B("n,k,j") with n a Hadamard (fused)
+    mode, leading in all three: TA's batched Summa with BatchedContractReduce
+    (expressions/cont_engine.h:684-1100, tile_op/batched_contract_reduce.h:
+    48-240, dist_eval/contraction_eval.h:96-187), where every element of the
+    fused mode is an independent product C_n = A_n * B_n (tensor.h:3211-3215).
+
+    Every element of n is one instance of the batched launch
+    (BatchedContraction): its cells are slices of the stored tiles, A tile
+    (n_t, i_t, k_t) giving the slice at offset + p * m_i * k_k for element p
+    of the tile, so nothing is copied in.  result() copies the instances'
+    result cells into (n, i, j) tiles with the permute kernel (identity
+    order).  threshold None: no screening (every result tile is formed, as
+    for dense arrays); a float screens per slice, which is finer than TA's
+    per-tile shape (SparseShape::gemm_batched)."""
+
+    def __init__(self, ctx, A, B, threshold=None, alpha=1.0):
+        ta, tb = A.trange.ranges, B.trange.ranges
+        if len(ta) != 3 or len(tb) != 3:
+            raise ValueError("FusedContraction: A is (n, i, k) and B is (n, k, j)")
+        if not (ta[0] == tb[0] and ta[2] == tb[1]):
+            raise ValueError("FusedContraction: the n and k tilings of A and B differ")
+        if A.pool.dtype != B.pool.dtype:
+            raise ValueError("FusedContraction: A and B differ in type")
+        self.ctx, self.A, self.B = ctx, A, B
+        trn, tri, trk, trj = ta[0], ta[1], ta[2], tb[2]
+        self.ranges = (trn, tri, trj)
+        nt, mt, kt, jt = trn.ntiles, tri.ntiles, trk.ntiles, trj.ntiles
+        me, ke, je = tri.extents, trk.extents, trj.extents
+        # element p of fused tile t: global element index, tile and position
+        self.elem_tile = np.repeat(np.arange(nt), trn.extents)
+        self.elem_pos = np.concatenate([np.arange(w) for w in trn.extents]) if nt else np.zeros(0, np.int64)
+        N = int(self.elem_tile.size)
+        self.N, self.Mt, self.Kt, self.Jt = N, mt, kt, jt
+        ao = A.offsets.reshape(nt, mt, kt)
+        bo = B.offsets.reshape(nt, kt, jt)
+        t, p = self.elem_tile, self.elem_pos
+        a_slice = (me[:, None] * ke[None, :]).astype(np.int64)   # mt x kt
+        b_slice = (ke[:, None] * je[None, :]).astype(np.int64)   # kt x jt
+        a_off = np.where(ao[t] >= 0, ao[t] + p[:, None, None] * a_slice[None], -1)
+        b_off = np.where(bo[t] >= 0, bo[t] + p[:, None, None] * b_slice[None], -1)
+        thr = 0.0 if threshold is None else float(threshold)
+        self.bc = BatchedContraction(ctx, max(N, 1), mt, jt, kt, np.tile(me, max(N, 1)),
+                                     np.tile(je, max(N, 1)), np.tile(ke, max(N, 1)),
+                                     A.pool, a_off.reshape(-1) if N else np.full(mt * kt, -1),
+                                     B.pool, b_off.reshape(-1) if N else np.full(kt * jt, -1),
+                                     threshold=thr, alpha=alpha)
+
+    def run(self):
+        self.bc.run()
+
+    def counters(self):
+        return self.bc.counters()
+
+    def result(self):
+        """The result DistArray over (n, i, j) (synchronises)."""
+        trn, tri, trj = self.ranges
+        nt, mt, jt = trn.ntiles, tri.ntiles, trj.ntiles
+        N = self.N
+        c_off = self.bc.c_off.cpu().numpy()[:N * mt * jt].reshape(N, mt, jt)
+        present = np.zeros((nt, mt, jt), bool)
+        for t in range(nt):
+            sel = self.elem_tile == t
+            if sel.any():
+                present[t] = (c_off[sel] >= 0).any(axis=0)
+        R = DistArray.allocate(TiledRange([trn, tri, trj]), present, dtype=self.A.pool.dtype,
+                               device=self.A.pool.device)
+        R.pool.zero_()
+        ro = R.offsets.reshape(nt, mt, jt)
+        me, je = tri.extents, trj.extents
+        n, i, j = np.nonzero(c_off >= 0)
+        if n.size:
+            t, p = self.elem_tile[n], self.elem_pos[n]
+            ext = np.stack([me[i], je[j]], axis=1)
+            dst = ro[t, i, j] + p * me[i] * je[j]
+            plan = PermutePlan(self.ctx, [0, 1], ext, c_off[n, i, j], dst)
+            if plan.volume:
+                plan.run(self.bc.C, R.pool)
+        return R
