@@ -159,3 +159,31 @@ def test_fused_mode_product_integer_exact(ctx, dtype):
     t = np.repeat(np.arange(3), trn.extents)
     expect = int(sum((ma[t[n]][:, :, None] & mb[t[n]][None, :, :]).sum() for n in range(8)))
     assert c.counters()["npairs"] == expect
+
+
+@pytest.mark.parametrize("target,left,right", [
+    ("n,i,j", "n,i,k", "n,k,j"),
+    ("i,n,j", "i,k,n", "j,n,k"),
+    ("j,i,n", "k,n,i", "n,j,k"),
+])
+def test_fused_contract_any_order(ctx, target, left, right):
+    # the fused index anywhere in the operands and the result: operands and
+    # result permuted by the permute kernel around the fused product; exact
+    # against numpy on integers
+    from tiledarray_amd import DistArray, TiledRange, TiledRange1
+    rng = np.random.default_rng(1600)
+    tr = {"n": TiledRange1([0, 2, 5]), "i": TiledRange1([0, 20, 45]),
+          "k": TiledRange1([0, 17, 40, 41]), "j": TiledRange1([0, 33, 50])}
+    L, R_, T = left.split(","), right.split(","), target.split(",")
+    A = DistArray.allocate(TiledRange([tr[x] for x in L]),
+                           rng.random(tuple(tr[x].ntiles for x in L)) < 0.8)
+    B = DistArray.allocate(TiledRange([tr[x] for x in R_]),
+                           rng.random(tuple(tr[x].ntiles for x in R_)) < 0.8)
+    for X in (A, B):
+        X.pool.copy_(torch.as_tensor(rng.integers(-9, 10, X.pool.numel()), dtype=torch.float64))
+    R, cnt = E.fused_contract(ctx, target, A, left, B, right, alpha=-1.0)
+    torch.cuda.synchronize()
+    ref = -np.einsum(f"{''.join(L)},{''.join(R_)}->{''.join(T)}", A.to_dense(), B.to_dense())
+    got = R.to_dense()
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got, ref)

@@ -973,3 +973,41 @@ class FusedContraction:
             if plan.volume:
                 plan.run(self.bc.C, R.pool)
         return R
+
+
+def fused_contract(ctx, target, A, left, B, right, alpha=1.0, threshold=None):
+    """target = alpha * A(left) * B(right) for rank-3 operands with one
+    Hadamard (fused) index n (in left, right and target), one contracted
+    index k and one free index per operand (i of A, j of B), in any order:
+    the operands are brought to (n, i, k) and (n, k, j) by the permute kernel
+    where their order differs (TA permutes `general` operands the same way,
+    tensor/permute.h:121-207), the product runs as FusedContraction, and the
+    result is permuted from (n, i, j) to the target order.  Returns
+    (result, counters)."""
+    L = [x.strip() for x in left.split(",")]
+    Rr = [x.strip() for x in right.split(",")]
+    T = [x.strip() for x in target.split(",")]
+    if not (len(L) == len(Rr) == len(T) == 3 and len(set(L)) == len(set(Rr)) == len(set(T)) == 3):
+        raise ValueError("fused_contract: three distinct indices per operand and result")
+    fused = [x for x in L if x in Rr and x in T]
+    inner = [x for x in L if x in Rr and x not in T]
+    free_a = [x for x in L if x not in Rr]
+    free_b = [x for x in Rr if x not in L]
+    if not (len(fused) == len(inner) == len(free_a) == len(free_b) == 1
+            and set(T) == {fused[0], free_a[0], free_b[0]}):
+        raise ValueError("fused_contract: needs one fused, one contracted and one free "
+                         "index per operand")
+    n, k, i, j = fused[0], inner[0], free_a[0], free_b[0]
+
+    def in_order(X, idx, want):
+        if idx == want:
+            return X
+        return permute_array(ctx, X, [want.index(x) for x in idx])  # TO convention
+
+    f = FusedContraction(ctx, in_order(A, L, [n, i, k]), in_order(B, Rr, [n, k, j]),
+                         threshold=threshold, alpha=alpha)
+    f.run()
+    R = f.result()
+    if T != [n, i, j]:
+        R = permute_array(ctx, R, [T.index(x) for x in (n, i, j)])
+    return R, f.counters()
