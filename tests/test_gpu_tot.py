@@ -165,15 +165,22 @@ def test_fused_mode_product_integer_exact(ctx, dtype):
     ("n,i,j", "n,i,k", "n,k,j"),
     ("i,n,j", "i,k,n", "j,n,k"),
     ("j,i,n", "k,n,i", "n,j,k"),
+    ("n,m,i,j", "n,i,k,m", "m,k,n,j"),     # two fused modes
+    ("n,i,j", "n,p,i,q", "q,n,p,j"),       # two contracted modes
+    ("i,n", "n,i,k", "k,n"),               # no free mode in B
+    ("n,i,j", "i,n", "n,j"),               # no contracted mode (Hadamard outer product)
 ])
 def test_fused_contract_any_order(ctx, target, left, right):
-    # the fused index anywhere in the operands and the result: operands and
-    # result permuted by the permute kernel around the fused product; exact
-    # against numpy on integers
+    # fused, contracted and free indices anywhere in the operands and the
+    # result, one or more of each (none for the contracted or a free group):
+    # operands and result permuted by the permute kernel around the fused
+    # product; exact against numpy on integers
     from tiledarray_amd import DistArray, TiledRange, TiledRange1
     rng = np.random.default_rng(1600)
     tr = {"n": TiledRange1([0, 2, 5]), "i": TiledRange1([0, 20, 45]),
-          "k": TiledRange1([0, 17, 40, 41]), "j": TiledRange1([0, 33, 50])}
+          "k": TiledRange1([0, 17, 40, 41]), "j": TiledRange1([0, 33, 50]),
+          "m": TiledRange1([0, 1, 3]), "p": TiledRange1([0, 6, 13]),
+          "q": TiledRange1([0, 4, 5, 11])}
     L, R_, T = left.split(","), right.split(","), target.split(",")
     A = DistArray.allocate(TiledRange([tr[x] for x in L]),
                            rng.random(tuple(tr[x].ntiles for x in L)) < 0.8)

@@ -895,39 +895,48 @@ class BatchedContraction:
 
 
 class FusedContraction:
-    """C("n,i,j") = alpha * A("n,i,k") * B("n,k,j") with n a Hadamard (fused)
-    mode, leading in all three: TA's batched Summa with BatchedContractReduce
-    (expressions/cont_engine.h:684-1100, tile_op/batched_contract_reduce.h:
-    48-240, dist_eval/contraction_eval.h:96-187), where every element of the
-    fused mode is an independent product C_n = A_n * B_n (tensor.h:3211-3215).
+    """C(F, I, J) = alpha * A(F, I, K) * B(F, K, J) with F Hadamard (fused)
+    modes, leading in all three, K contracted and I, J free mode groups (the
+    groups hold nf, ni, nk, nj modes; ni, nk or nj may be 0): TA's batched
+    Summa with BatchedContractReduce (expressions/cont_engine.h:684-1100,
+    tile_op/batched_contract_reduce.h:48-240, dist_eval/contraction_eval.h:
+    96-187), where every element of the fused modes is an independent
+    product C_f = A_f * B_f (tensor.h:3211-3215).
 
-    Every element of n is one instance of the batched launch
-    (BatchedContraction): its cells are slices of the stored tiles, A tile
-    (n_t, i_t, k_t) giving the slice at offset + p * m_i * k_k for element p
-    of the tile, so nothing is copied in.  result() copies the instances'
-    result cells into (n, i, j) tiles with the permute kernel (identity
-    order).  threshold None: no screening (every result tile is formed, as
-    for dense arrays); a float screens per slice, which is finer than TA's
-    per-tile shape (SparseShape::gemm_batched)."""
+    Each group's modes are fused into one (their tile combinations in
+    row-major order, as GemmHelper fuses modes).  Every fused element is one
+    instance of the batched launch (BatchedContraction): its cells are slices
+    of the stored tiles, A tile (f, i, k) giving the slice at offset
+    + p * m_i * k_k for element p of fused tile f, so nothing is copied in.
+    result() copies the instances' result cells into (F, I, J) tiles with the
+    permute kernel (identity order).  threshold None: no screening (every
+    result tile is formed, as for dense arrays); a float screens per slice,
+    which is finer than TA's per-tile shape (SparseShape::gemm_batched)."""
 
-    def __init__(self, ctx, A, B, threshold=None, alpha=1.0):
+    def __init__(self, ctx, A, B, groups=(1, 1, 1, 1), threshold=None, alpha=1.0):
+        nf, ni, nk, nj = (int(g) for g in groups)
         ta, tb = A.trange.ranges, B.trange.ranges
-        if len(ta) != 3 or len(tb) != 3:
-            raise ValueError("FusedContraction: A is (n, i, k) and B is (n, k, j)")
-        if not (ta[0] == tb[0] and ta[2] == tb[1]):
-            raise ValueError("FusedContraction: the n and k tilings of A and B differ")
+        if nf < 1 or len(ta) != nf + ni + nk or len(tb) != nf + nk + nj:
+            raise ValueError("FusedContraction: A is (F, I, K) and B is (F, K, J), F not empty")
+        if not (all(x == y for x, y in zip(ta[:nf], tb[:nf]))
+                and all(x == y for x, y in zip(ta[nf + ni:], tb[nf:nf + nk]))):
+            raise ValueError("FusedContraction: the fused or contracted tilings of A and B differ")
         if A.pool.dtype != B.pool.dtype:
             raise ValueError("FusedContraction: A and B differ in type")
         self.ctx, self.A, self.B = ctx, A, B
-        trn, tri, trk, trj = ta[0], ta[1], ta[2], tb[2]
-        self.ranges = (trn, tri, trj)
-        nt, mt, kt, jt = trn.ntiles, tri.ntiles, trk.ntiles, trj.ntiles
-        me, ke, je = tri.extents, trk.extents, trj.extents
+        self.ranges = list(ta[:nf]) + list(ta[nf:nf + ni]) + list(tb[nf + nk:])
+        fe = _fused_extents(ta[:nf])
+        me = _fused_extents(ta[nf:nf + ni])
+        ke = _fused_extents(ta[nf + ni:])
+        je = _fused_extents(tb[nf + nk:])
+        nt, mt, kt, jt = fe.size, me.size, ke.size, je.size
         # element p of fused tile t: global element index, tile and position
-        self.elem_tile = np.repeat(np.arange(nt), trn.extents)
-        self.elem_pos = np.concatenate([np.arange(w) for w in trn.extents]) if nt else np.zeros(0, np.int64)
+        self.elem_tile = np.repeat(np.arange(nt), fe)
+        self.elem_pos = (np.concatenate([np.arange(w) for w in fe]) if fe.sum()
+                         else np.zeros(0, np.int64))
         N = int(self.elem_tile.size)
         self.N, self.Mt, self.Kt, self.Jt = N, mt, kt, jt
+        self.me, self.je = me, je
         ao = A.offsets.reshape(nt, mt, kt)
         bo = B.offsets.reshape(nt, kt, jt)
         t, p = self.elem_tile, self.elem_pos
@@ -936,9 +945,10 @@ class FusedContraction:
         a_off = np.where(ao[t] >= 0, ao[t] + p[:, None, None] * a_slice[None], -1)
         b_off = np.where(bo[t] >= 0, bo[t] + p[:, None, None] * b_slice[None], -1)
         thr = 0.0 if threshold is None else float(threshold)
-        self.bc = BatchedContraction(ctx, max(N, 1), mt, jt, kt, np.tile(me, max(N, 1)),
-                                     np.tile(je, max(N, 1)), np.tile(ke, max(N, 1)),
-                                     A.pool, a_off.reshape(-1) if N else np.full(mt * kt, -1),
+        nb = max(N, 1)
+        self.bc = BatchedContraction(ctx, nb, mt, jt, kt, np.tile(me, nb), np.tile(je, nb),
+                                     np.tile(ke, nb), A.pool,
+                                     a_off.reshape(-1) if N else np.full(mt * kt, -1),
                                      B.pool, b_off.reshape(-1) if N else np.full(kt * jt, -1),
                                      threshold=thr, alpha=alpha)
 
@@ -949,21 +959,21 @@ class FusedContraction:
         return self.bc.counters()
 
     def result(self):
-        """The result DistArray over (n, i, j) (synchronises)."""
-        trn, tri, trj = self.ranges
-        nt, mt, jt = trn.ntiles, tri.ntiles, trj.ntiles
-        N = self.N
+        """The result DistArray over (F, I, J) (synchronises)."""
+        N, mt, jt = self.N, self.Mt, self.Jt
+        trange = TiledRange(self.ranges)
+        ntiles_f = trange.ntiles // (mt * jt)
         c_off = self.bc.c_off.cpu().numpy()[:N * mt * jt].reshape(N, mt, jt)
-        present = np.zeros((nt, mt, jt), bool)
-        for t in range(nt):
+        present = np.zeros((ntiles_f, mt, jt), bool)
+        for t in range(ntiles_f):
             sel = self.elem_tile == t
             if sel.any():
                 present[t] = (c_off[sel] >= 0).any(axis=0)
-        R = DistArray.allocate(TiledRange([trn, tri, trj]), present, dtype=self.A.pool.dtype,
+        R = DistArray.allocate(trange, present.reshape(-1), dtype=self.A.pool.dtype,
                                device=self.A.pool.device)
         R.pool.zero_()
-        ro = R.offsets.reshape(nt, mt, jt)
-        me, je = tri.extents, trj.extents
+        ro = R.offsets.reshape(ntiles_f, mt, jt)
+        me, je = self.me, self.je
         n, i, j = np.nonzero(c_off >= 0)
         if n.size:
             t, p = self.elem_tile[n], self.elem_pos[n]
@@ -976,38 +986,37 @@ class FusedContraction:
 
 
 def fused_contract(ctx, target, A, left, B, right, alpha=1.0, threshold=None):
-    """target = alpha * A(left) * B(right) for rank-3 operands with one
-    Hadamard (fused) index n (in left, right and target), one contracted
-    index k and one free index per operand (i of A, j of B), in any order:
-    the operands are brought to (n, i, k) and (n, k, j) by the permute kernel
-    where their order differs (TA permutes `general` operands the same way,
-    tensor/permute.h:121-207), the product runs as FusedContraction, and the
-    result is permuted from (n, i, j) to the target order.  Returns
-    (result, counters)."""
+    """target = alpha * A(left) * B(right) where some indices are Hadamard
+    (fused: in left, right and target), some contracted (in left and right
+    only) and the rest free, in any order: the operands are brought to
+    (F, I, K) and (F, K, J) by the permute kernel where their order differs
+    (TA permutes `general` operands the same way, tensor/permute.h:121-207),
+    with F, I and J in target order and K in left order; the product runs as
+    FusedContraction, and the result is permuted from (F, I, J) to the
+    target order.  Returns (result, counters)."""
     L = [x.strip() for x in left.split(",")]
     Rr = [x.strip() for x in right.split(",")]
     T = [x.strip() for x in target.split(",")]
-    if not (len(L) == len(Rr) == len(T) == 3 and len(set(L)) == len(set(Rr)) == len(set(T)) == 3):
-        raise ValueError("fused_contract: three distinct indices per operand and result")
-    fused = [x for x in L if x in Rr and x in T]
-    inner = [x for x in L if x in Rr and x not in T]
-    free_a = [x for x in L if x not in Rr]
-    free_b = [x for x in Rr if x not in L]
-    if not (len(fused) == len(inner) == len(free_a) == len(free_b) == 1
-            and set(T) == {fused[0], free_a[0], free_b[0]}):
-        raise ValueError("fused_contract: needs one fused, one contracted and one free "
-                         "index per operand")
-    n, k, i, j = fused[0], inner[0], free_a[0], free_b[0]
+    if len(set(L)) != len(L) or len(set(Rr)) != len(Rr) or len(set(T)) != len(T):
+        raise ValueError("fused_contract: an index repeats within an operand or the result")
+    F = [x for x in T if x in L and x in Rr]
+    I = [x for x in T if x in L and x not in Rr]
+    J = [x for x in T if x in Rr and x not in L]
+    K = [x for x in L if x in Rr and x not in T]
+    if not F or set(T) != set(F + I + J) or set(L) != set(F + I + K) or set(Rr) != set(F + K + J):
+        raise ValueError("fused_contract: needs a fused index, and every index of the result "
+                         "in an operand")
 
     def in_order(X, idx, want):
         if idx == want:
             return X
         return permute_array(ctx, X, [want.index(x) for x in idx])  # TO convention
 
-    f = FusedContraction(ctx, in_order(A, L, [n, i, k]), in_order(B, Rr, [n, k, j]),
-                         threshold=threshold, alpha=alpha)
+    f = FusedContraction(ctx, in_order(A, L, F + I + K), in_order(B, Rr, F + K + J),
+                         groups=(len(F), len(I), len(K), len(J)), threshold=threshold,
+                         alpha=alpha)
     f.run()
     R = f.result()
-    if T != [n, i, j]:
-        R = permute_array(ctx, R, [T.index(x) for x in (n, i, j)])
+    if T != F + I + J:
+        R = permute_array(ctx, R, [T.index(x) for x in F + I + J])
     return R, f.counters()
