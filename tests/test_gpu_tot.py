@@ -194,3 +194,54 @@ def test_fused_contract_any_order(ctx, target, left, right):
     got = R.to_dense()
     assert got.shape == ref.shape
     np.testing.assert_array_equal(got, ref)
+
+
+def test_fused_contract_sparse_tile_screening(ctx):
+    # the fused product with screened shapes: the result estimate is the
+    # shape GEMM of each fused tile-slab (SparseShape::gemm_batched), so a
+    # result tile exists exactly when the oracle's per-slab estimate reaches
+    # the threshold, and its values are the product over the operand tiles
+    # the shapes keep (exact on integers scaled by powers of two)
+    from tiledarray_amd import DistArray, TiledRange, TiledRange1
+    rng = np.random.default_rng(1700)
+    thr = 1e-6
+    trn, tri = TiledRange1([0, 2, 5, 6]), TiledRange1([0, 20, 45, 51])
+    trk, trj = TiledRange1([0, 17, 40, 41]), TiledRange1([0, 33, 50])
+    A = DistArray.allocate(TiledRange([trn, tri, trk]), rng.random((3, 3, 3)) < 0.9)
+    B = DistArray.allocate(TiledRange([trn, trk, trj]), rng.random((3, 3, 2)) < 0.9)
+    for X in (A, B):
+        X.pool.copy_(torch.as_tensor(rng.integers(-9, 10, X.pool.numel()), dtype=torch.float64))
+        # some tiles scaled far down: their shapes fall below the threshold
+        vols = X.trange.tile_volumes()
+        for t in np.nonzero(X.offsets >= 0)[0]:
+            if rng.random() < 0.35:
+                o = X.offsets[t]
+                X.pool[o:o + vols[t]] *= 2.0 ** -40
+    E.make_sparse(ctx, A, thr)
+    E.make_sparse(ctx, B, thr)
+    f = E.FusedContraction(ctx, A, B, threshold=thr)
+    f.run()
+    R = f.result()
+    torch.cuda.synchronize()
+    sa = A.shape_norms.cpu().numpy().reshape(3, 3, 3)
+    sb = B.shape_norms.cpu().numpy().reshape(3, 3, 2)
+    kz = oracle.k_sizes([trk.extents])
+    present = np.zeros((3, 3, 2), bool)
+    for h in range(3):
+        est, _ = oracle.shape_gemm(sa[h], sb[h], 0, 0, 3, 2, 3, kz, 1.0, thr)
+        present[h] = est >= thr
+    assert 0 < present.sum() < present.size
+    assert np.array_equal(R.offsets.reshape(3, 3, 2) >= 0, present)
+    # reference over the operand tiles the shapes keep
+    Ad, Bd = A.to_dense(), B.to_dense()
+    for X, D, s, tr3 in ((A, Ad, sa, (trn, tri, trk)), (B, Bd, sb, (trn, trk, trj))):
+        for idx in zip(*np.nonzero(s < thr)):
+            sl = tuple(slice(r.boundaries[a], r.boundaries[a + 1]) for r, a in zip(tr3, idx))
+            D[sl] = 0.0
+    ref = np.einsum("nik,nkj->nij", Ad, Bd)
+    got = R.to_dense()
+    for h, i, j in zip(*np.nonzero(present)):
+        sl = (slice(trn.boundaries[h], trn.boundaries[h + 1]),
+              slice(tri.boundaries[i], tri.boundaries[i + 1]),
+              slice(trj.boundaries[j], trj.boundaries[j + 1]))
+        np.testing.assert_array_equal(got[sl], ref[sl])

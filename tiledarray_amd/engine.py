@@ -909,9 +909,16 @@ class FusedContraction:
     of the stored tiles, A tile (f, i, k) giving the slice at offset
     + p * m_i * k_k for element p of fused tile f, so nothing is copied in.
     result() copies the instances' result cells into (F, I, J) tiles with the
-    permute kernel (identity order).  threshold None: no screening (every
-    result tile is formed, as for dense arrays); a float screens per slice,
-    which is finer than TA's per-tile shape (SparseShape::gemm_batched)."""
+    permute kernel (identity order).
+
+    threshold None: no screening, every result tile is formed (dense arrays).
+    threshold t: A and B carry their shapes (make_sparse), and the screening
+    is TA's per tile, SparseShape::gemm_batched (sparse_shape.h:1695-1880):
+    the result estimate is the shape GEMM of each fused tile-slab (batched
+    over the fused tiles, contracted rows scaled by the contracted tile
+    volumes), tiles below t are zero, and the operand tile norms and the
+    estimate are replicated over every element of their fused tile, so a
+    pair or a result cell is kept exactly when its tile is."""
 
     def __init__(self, ctx, A, B, groups=(1, 1, 1, 1), threshold=None, alpha=1.0):
         nf, ni, nk, nj = (int(g) for g in groups)
@@ -944,16 +951,49 @@ class FusedContraction:
         b_slice = (ke[:, None] * je[None, :]).astype(np.int64)   # kt x jt
         a_off = np.where(ao[t] >= 0, ao[t] + p[:, None, None] * a_slice[None], -1)
         b_off = np.where(bo[t] >= 0, bo[t] + p[:, None, None] * b_slice[None], -1)
-        thr = 0.0 if threshold is None else float(threshold)
+        thr = 0.0 if threshold is None else float(np.float32(threshold))
         nb = max(N, 1)
         self.bc = BatchedContraction(ctx, nb, mt, jt, kt, np.tile(me, nb), np.tile(je, nb),
                                      np.tile(ke, nb), A.pool,
                                      a_off.reshape(-1) if N else np.full(mt * kt, -1),
                                      B.pool, b_off.reshape(-1) if N else np.full(kt * jt, -1),
                                      threshold=thr, alpha=alpha)
+        self.tile_shapes = threshold is not None
+        if self.tile_shapes:
+            if A.shape_norms is None or B.shape_norms is None:
+                raise ValueError("FusedContraction: a threshold needs A and B with shapes "
+                                 "(make_sparse)")
+            dev = A.pool.device
+            self.nt, self.alpha, self.threshold = nt, float(alpha), thr
+            self.est = torch.zeros(nt * mt * jt, dtype=torch.float32, device=dev)
+            kz = k_sizes(ta[nf + ni:]) if nk else np.ones(1, np.float32)
+            self.ksz = torch.as_tensor(np.tile(kz, nt), device=dev)
+            self.d_elem_tile = torch.as_tensor(self.elem_tile, device=dev)
+            # operand tiles are screened by their own shapes' thresholds
+            self.bc._desc.a_threshold = float(A.shape_threshold)
+            self.bc._desc.b_threshold = float(B.shape_threshold)
+
+    def _tile_shape_step(self):
+        """The per-tile result estimate (batched over the fused tiles) and
+        the tile norms and estimate replicated over the fused elements."""
+        mt, kt, jt, nt = self.Mt, self.Kt, self.Jt, self.nt
+        call("tamx_shape_gemm_f32", self.ctx.ptr, self.A.shape_norms.data_ptr(),
+             self.B.shape_norms.data_ptr(), 0, 0, mt, jt, kt, nt, self.ksz.data_ptr(),
+             ctypes.c_float(abs(self.alpha)), ctypes.c_float(self.threshold),
+             self.est.data_ptr(), None, self.ctx.stream())
+        if self.N:
+            t = self.d_elem_tile
+            self.bc.a_norms.copy_(self.A.shape_norms.view(nt, mt, kt)[t].reshape(-1))
+            self.bc.b_norms.copy_(self.B.shape_norms.view(nt, kt, jt)[t].reshape(-1))
+            self.bc.c_norms.copy_(self.est.view(nt, mt, jt)[t].reshape(-1))
 
     def run(self):
-        self.bc.run()
+        if self.tile_shapes:
+            self._tile_shape_step()
+            self.bc.worklist_step()
+            self.bc.gemm_step()
+        else:
+            self.bc.run()
 
     def counters(self):
         return self.bc.counters()
