@@ -245,3 +245,40 @@ def test_fused_contract_sparse_tile_screening(ctx):
               slice(tri.boundaries[i], tri.boundaries[i + 1]),
               slice(trj.boundaries[j], trj.boundaries[j + 1]))
         np.testing.assert_array_equal(got[sl], ref[sl])
+
+
+@pytest.mark.parametrize("precision", ["i8x3", "bf16x3"])
+def test_tot_split_precision_integer_exact(ctx, precision):
+    # the batched launch with split-precision GEMMs: every cell is one tile of
+    # the limb split (ordinals as in the work list's pairs); integer cells
+    # whose entries fit the first limb reproduce the oracle bit for bit
+    w = W.tot_batch(nbatch=24, dtype=torch.float64, integer=True, seed=9, fill=0.5)
+    g = w["grid"]
+    c = E.BatchedContraction(ctx, 24, g, g, g, w["a_dims"], w["b_dims"], w["c_dims"],
+                             w["A"], w["a_off"], w["B"], w["b_off"], precision=precision)
+    c.run()
+    torch.cuda.synchronize()
+    Cref, est, c_off = oracle_batch(w, c)
+    np.testing.assert_array_equal(c_off >= 0, est >= c.threshold)
+    m = used_mask(w, c, c_off)
+    assert m.any()
+    np.testing.assert_array_equal(c.C.cpu().numpy()[m], Cref[m])
+
+
+def test_fused_contract_split_precision(ctx):
+    # the fused product through the limb split (5 INT8 limbs) on random data,
+    # against the FP64 fused product: within the five-limb error bound
+    from tiledarray_amd import DistArray, TiledRange, TiledRange1
+    rng = np.random.default_rng(1800)
+    tr = {"n": TiledRange1([0, 2, 5]), "i": TiledRange1([0, 40, 90]),
+          "k": TiledRange1([0, 70, 150]), "j": TiledRange1([0, 64, 100])}
+    A = DistArray.allocate(TiledRange([tr[x] for x in "nik"]))
+    B = DistArray.allocate(TiledRange([tr[x] for x in "nkj"]))
+    for X in (A, B):
+        X.pool.copy_(torch.as_tensor(rng.uniform(-1, 1, X.pool.numel())))
+    R64, _ = E.fused_contract(ctx, "n,i,j", A, "n,i,k", B, "n,k,j")
+    R8, _ = E.fused_contract(ctx, "n,i,j", A, "n,i,k", B, "n,k,j", precision="i8x5")
+    torch.cuda.synchronize()
+    ref, got = R64.to_dense(), R8.to_dense()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert 0 < err <= 2e-9, err

@@ -774,7 +774,7 @@ class BatchedContraction:
     for a single contraction, batched."""
 
     def __init__(self, ctx, nbatch, Mt, Nt, Kt, mdims, ndims, kdims, A, a_off, B, b_off,
-                 threshold=FLOAT_EPS, alpha=1.0, tile=None):
+                 threshold=FLOAT_EPS, alpha=1.0, tile=None, precision=None):
         self.ctx = ctx
         dev = A.device
         self.nbatch, self.Mt, self.Nt, self.Kt = int(nbatch), int(Mt), int(Nt), int(Kt)
@@ -826,6 +826,37 @@ class BatchedContraction:
         d.c_capacity = self.c_capacity
         self._desc = d
         self.tile_cfg = None if tile is None else _lib.GemmConfig(*(tuple(tile) + (16, 0))[:4])
+        # split precision (FP64 cells): every cell is one tile of the limb
+        # split, with the ordinal the work list's pairs carry (b, i, k) and
+        # (b, k, j)
+        self.precision = precision or ("f64" if self.dtype == torch.float64 else "f32")
+        self.nslices = 0
+        if self.precision.startswith(("bf16x", "i8x")):
+            if self.dtype != torch.float64:
+                raise ValueError("BatchedContraction: split precisions need FP64 cells")
+            self.nslices = int(self.precision.split("x")[-1])
+            self.limb_type = _lib.LIMB_I8 if self.precision.startswith("i8x") else _lib.LIMB_BF16
+            a_rows = np.repeat(md, Kt, axis=1).reshape(-1)
+            a_cols = np.tile(kd, (1, Mt)).reshape(-1)
+            b_rows = np.repeat(kd, Nt, axis=1).reshape(-1)
+            b_cols = np.tile(nd, (1, Kt)).reshape(-1)
+            self.a_split = self._split_plan(self.a_off_h, a_rows, a_cols, False)
+            self.b_split = self._split_plan(self.b_off_h, b_rows, b_cols, True)
+        elif self.precision not in ("f64", "f32"):
+            raise ValueError(f"BatchedContraction: unknown precision {self.precision!r}")
+
+    def _split_plan(self, offs, rows, cols, is_right):
+        offs = np.ascontiguousarray(offs, np.int64)
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        p = ctypes.c_void_p()
+        call("tamx_split_plan_create_typed", self.ctx.ptr, self.limb_type, self.nslices,
+             1 if is_right else 0, 0, offs.size, offs.ctypes.data, rows.ctypes.data,
+             cols.ctypes.data, ctypes.byref(p))
+        n = int(lib().tamx_split_plan_limb_elems(p))
+        dtype = torch.uint8 if self.limb_type == _lib.LIMB_I8 else torch.bfloat16
+        return _SplitOperand(p, torch.zeros(max(1, n), dtype=dtype, device=self.A.device),
+                             offs.size)
 
     def shape_step(self):
         """Cell norms, inverse-volume scaling, batched result estimate."""
@@ -850,7 +881,13 @@ class BatchedContraction:
     def gemm_step(self):
         s = self.ctx.stream()
         cfg = None if self.tile_cfg is None else ctypes.byref(self.tile_cfg)
-        if self.dtype == torch.float64:
+        if self.nslices:
+            for X, sp in ((self.A, self.a_split), (self.B, self.b_split)):
+                call("tamx_split_f64", self.ctx.ptr, sp.ptr, X.data_ptr(), sp.limbs.data_ptr(), s)
+            call("tamx_grouped_gemm_split_config", self.ctx.ptr, self.wl.ptr,
+                 ctypes.c_double(self.alpha), self.a_split.ptr, self.a_split.limbs.data_ptr(),
+                 self.b_split.ptr, self.b_split.limbs.data_ptr(), self.C.data_ptr(), cfg, s)
+        elif self.dtype == torch.float64:
             call("tamx_grouped_gemm_f64_config", self.ctx.ptr, self.wl.ptr, 0, 0,
                  ctypes.c_double(self.alpha), self.A.data_ptr(), self.B.data_ptr(),
                  self.C.data_ptr(), cfg, s)
@@ -920,7 +957,8 @@ class FusedContraction:
     estimate are replicated over every element of their fused tile, so a
     pair or a result cell is kept exactly when its tile is."""
 
-    def __init__(self, ctx, A, B, groups=(1, 1, 1, 1), threshold=None, alpha=1.0):
+    def __init__(self, ctx, A, B, groups=(1, 1, 1, 1), threshold=None, alpha=1.0,
+                 precision=None):
         nf, ni, nk, nj = (int(g) for g in groups)
         ta, tb = A.trange.ranges, B.trange.ranges
         if nf < 1 or len(ta) != nf + ni + nk or len(tb) != nf + nk + nj:
@@ -957,7 +995,7 @@ class FusedContraction:
                                      np.tile(ke, nb), A.pool,
                                      a_off.reshape(-1) if N else np.full(mt * kt, -1),
                                      B.pool, b_off.reshape(-1) if N else np.full(kt * jt, -1),
-                                     threshold=thr, alpha=alpha)
+                                     threshold=thr, alpha=alpha, precision=precision)
         self.tile_shapes = threshold is not None
         if self.tile_shapes:
             if A.shape_norms is None or B.shape_norms is None:
@@ -1025,7 +1063,7 @@ class FusedContraction:
         return R
 
 
-def fused_contract(ctx, target, A, left, B, right, alpha=1.0, threshold=None):
+def fused_contract(ctx, target, A, left, B, right, alpha=1.0, threshold=None, precision=None):
     """target = alpha * A(left) * B(right) where some indices are Hadamard
     (fused: in left, right and target), some contracted (in left and right
     only) and the rest free, in any order: the operands are brought to
@@ -1054,7 +1092,7 @@ def fused_contract(ctx, target, A, left, B, right, alpha=1.0, threshold=None):
 
     f = FusedContraction(ctx, in_order(A, L, F + I + K), in_order(B, Rr, F + K + J),
                          groups=(len(F), len(I), len(K), len(J)), threshold=threshold,
-                         alpha=alpha)
+                         alpha=alpha, precision=precision)
     f.run()
     R = f.result()
     if T != F + I + J:
