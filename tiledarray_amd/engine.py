@@ -771,7 +771,9 @@ class BatchedContraction:
     Every instance has its own Mt x Kt and Kt x Nt cell grids (the same
     grid sizes) and its own inner extents; shapes are built from cell norms
     (float, inverse-volume scaled) and screened with `threshold`, exactly as
-    for a single contraction, batched."""
+    for a single contraction, batched.  precision: None (the cells' type,
+    FP64 or FP32 MFMA) or "i8xS" / "bf16xS" (FP64 cells split into S INT8 or
+    BF16 limbs, each cell one tile of the split plan)."""
 
     def __init__(self, ctx, nbatch, Mt, Nt, Kt, mdims, ndims, kdims, A, a_off, B, b_off,
                  threshold=FLOAT_EPS, alpha=1.0, tile=None, precision=None):
