@@ -165,3 +165,16 @@ def test_plan_constructors_reject_bad_arguments_before_touching_a_device():
                                      ctypes.byref(p)) == _lib.TAMX_ERR_BAD_ARG
     assert p.value is None
     assert L.tamx_split_plan_limb_type(None) == _lib.LIMB_BF16
+
+
+@pytest.mark.parametrize("target,left,right", [
+    ("i,j", "i,k", "k,j"),          # no fused index: a plain contraction
+    ("n,i,j", "n,k,k", "n,k,j"),    # an index repeated within an operand
+    ("n,i,i", "n,i,k", "n,k,j"),    # a repeated result index
+    ("n,i,j,z", "n,i,k", "n,k,j"),  # a result index in no operand
+])
+def test_fused_contract_rejects_bad_index_patterns(target, left, right):
+    # the index analysis runs before any device work
+    from tiledarray_amd import engine as E
+    with pytest.raises(ValueError):
+        E.fused_contract(None, target, None, left, None, right)
