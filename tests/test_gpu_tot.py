@@ -282,3 +282,34 @@ def test_fused_contract_split_precision(ctx):
     ref, got = R64.to_dense(), R8.to_dense()
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert 0 < err <= 2e-9, err
+
+
+def test_fused_contract_screened_split_matches_fp64_pattern(ctx):
+    # per-tile screening with the limb split: the same result tiles as the
+    # screened FP64 product, values within the five-limb bound
+    from tiledarray_amd import DistArray, TiledRange, TiledRange1
+    rng = np.random.default_rng(1900)
+    thr = 1e-6
+    tr = {"n": TiledRange1([0, 2, 5]), "i": TiledRange1([0, 40, 90, 100]),
+          "k": TiledRange1([0, 70, 150]), "j": TiledRange1([0, 64, 100])}
+    A = DistArray.allocate(TiledRange([tr[x] for x in "kni"]))
+    B = DistArray.allocate(TiledRange([tr[x] for x in "njk"]))
+    for X in (A, B):
+        X.pool.copy_(torch.as_tensor(rng.uniform(-1, 1, X.pool.numel())))
+        vols = X.trange.tile_volumes()
+        for t in range(X.trange.ntiles):
+            if rng.random() < 0.3:
+                o = X.offsets[t]
+                X.pool[o:o + vols[t]] *= 1e-9
+    E.make_sparse(ctx, A, thr)
+    E.make_sparse(ctx, B, thr)
+    R64, c64 = E.fused_contract(ctx, "i,n,j", A, "k,n,i", B, "n,j,k", threshold=thr)
+    R8, c8 = E.fused_contract(ctx, "i,n,j", A, "k,n,i", B, "n,j,k", threshold=thr,
+                              precision="i8x5")
+    torch.cuda.synchronize()
+    assert c64["npairs"] == c8["npairs"] and c64["ntasks"] == c8["ntasks"]
+    assert np.array_equal(R64.offsets >= 0, R8.offsets >= 0)
+    assert 0 < (R64.offsets >= 0).sum() < R64.offsets.size
+    ref, got = R64.to_dense(), R8.to_dense()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert 0 < err <= 2e-9, err
