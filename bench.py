@@ -31,9 +31,81 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes, each scaled by what the counter reads for a plain copy
 # of 1 GiB (524302 FETCH_SIZE units read, 1048576 WRITE_SIZE units written:
-# 2 KiB and 1 KiB per unit).  profiles/ holds the summaries.  Keyed by
-# workload, precision and macro tile.
-TRAFFIC_BYTES = {("blocksparse", "f64", (64, 64, 16)): 3785074 * 2048 + 942487 * 1024}
+# 2 KiB and 1 KiB per unit).  Keyed by workload, precision and macro tile,
+# with the profile summary the counters come from; a line whose
+# configuration has no entry reports traffic null.
+TRAFFIC_BYTES = {
+    ("blocksparse", "f64", (64, 64, 16)): (3785074 * 2048 + 942487 * 1024,
+                                           "profiles/grouped_gemm_f64_blocksparse16384.md"),
+}
+
+
+def traffic(workload, precision, tile):
+    """(bytes, source) of the profiled HBM traffic of this configuration, or
+    (None, None)."""
+    hit = TRAFFIC_BYTES.get((workload, precision, tuple(tile) if tile else None))
+    return hit if hit else (None, None)
+
+
+def host_cpu():
+    """Threads this process may run on, the machine's CPU count and model."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads_usable": usable, "nproc": os.cpu_count(), "model": model}
+
+
+def cpu_threads(args):
+    return args.cpu_threads or host_cpu()["threads_usable"]
+
+
+def median_time(fn, reps=3, warmup=1):
+    """Median wall time (s) of `reps` calls of fn after `warmup` calls; also
+    the last call's return value."""
+    import statistics
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), out
+
+
+def limb_products(precision):
+    """MFMA products per FP64-equivalent product and the pipe's published
+    peak for a split precision."""
+    if precision == "f32x2":
+        return 3, PEAK_FP32_TFLOPS, "FP32 MFMA"
+    s = int(precision.split("x")[-1])
+    if precision.startswith("i8"):
+        return s * (s + 1) // 2, PEAK_INT8_TOPS, "INT8 MFMA"
+    return s * (s + 1) // 2, PEAK_BF16_TFLOPS, "BF16 MFMA"
+
+
+def split_roofline(precision, flops, gemm_ms):
+    """The split GEMM against its own pipe: limb products (FP64-equivalent
+    flops times products per FP64 product) per second over the published
+    peak, and the FP64-equivalent rate against the FP64 peak."""
+    prods, peak, pipe = limb_products(precision)
+    limb = prods * flops / (gemm_ms * 1e-3) / 1e12
+    eq = flops / (gemm_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "pipe": pipe, "products_per_fp64_product": prods,
+            "achieved": round(limb, 3), "peak": peak, "unit": "TFLOP/s" if "FP" in pipe or
+            "BF16" in pipe else "TOP/s", "frac": round(limb / peak, 4),
+            "fp64_equivalent_frac": round(eq / PEAK_FP64_TFLOPS, 4)}
 
 
 def parse():
@@ -59,6 +131,8 @@ def parse():
     p.add_argument("--no-split", action="store_true", help="skip the split-precision runs")
     p.add_argument("--split", default="bf16x3,bf16x4,bf16x5,i8x4,i8x5,f32x2",
                    help="split precisions measured beside the FP64 run")
+    p.add_argument("--ladder-split", default="bf16x3,f32x2,i8x4,i8x5",
+                   help="ladder: split precisions measured beside the main precision")
     p.add_argument("--split-tile", default=None,
                    help="macro tile BMxBN of the split GEMM (64x64, 128x64 or 64x128)")
     p.add_argument("--v-permute", default="auto", choices=["auto", "copy", "map"],
@@ -231,6 +305,7 @@ def main():
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     peak = (PEAK_FP64_TFLOPS if args.precision == "f64"
             else PEAK_INT8_TOPS if args.precision.startswith("i8") else PEAK_BF16_TFLOPS)
+    tr_bytes, tr_src = traffic(args.workload, args.precision, used_tile or [64, 64, 16])
 
     out = {
         "metric": METRIC,
@@ -264,11 +339,13 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": TRAFFIC_BYTES.get((args.workload, args.precision,
-                                          tuple(used_tile or [64, 64, 16]))),
+            "traffic": tr_bytes,
+            "traffic_source": tr_src,
             "avg_launch_ms": round(gemm_ms, 4),
         },
     }
+    if sparse:
+        out["config"]["near_threshold_estimates"] = near_threshold(ctx, c)
 
     ref = None
     if not args.no_cpu:
@@ -301,12 +378,21 @@ def main():
                 "value": round(flops / (ms_s * 1e-3) / 1e12, 3),
                 "ms_per_step": round(ms_s, 4),
                 "gemm_tflops": round(flops / (gemm_s * 1e-3) / 1e12, 3),
+                "gemm_ms": round(gemm_s, 4),
                 "speedup_vs_f64": round(ms / ms_s, 3),
                 "macro_tile": list(st) if st else [64, 64],
                 "rel_err_frobenius": float(np.linalg.norm(g - r) / rn),
                 "rel_err_max": float(np.abs(g - r).max() / rmax),
                 "error_against": "oracle" if ref is not None else "fp64 result",
+                "roofline": split_roofline(prec, flops, gemm_s),
             }
+            if cs.nslices:
+                # the limb split: every stored FP64 element read once, S
+                # limbs of the padded limb layout written
+                mx = (maxabs[id(A)], maxabs[id(B)]) if sparse else None
+                nbytes = split_bytes(cs, A, B)
+                split[prec]["bandwidth"] = bandwidth(
+                    {"split_limbs": (nbytes, time_kernel(lambda: cs.split_step(mx)))})
             del cs
         out["split_precision"] = split
     if sparse:
@@ -330,11 +416,15 @@ def bench_ladder(args):
     split), the result estimate, the work list and the grouped GEMM with the
     result transpose fused into its store.  V is permuted into a copy in the
     step (as the reference permutes on read) or read through the
-    permutation's index map by the limb split or the GEMM (--v-permute)."""
+    permutation's index map by the limb split or the GEMM (--v-permute).
+
+    The line's precision (--precision, FP64 by default) is the main entry;
+    the split precisions of --split run on the same inputs beside it, each
+    with its speed-up and its max and Frobenius error against the oracle on
+    a sample of whole result tile columns."""
     import numpy as np
     import torch
 
-    import oracle
     from tiledarray_amd import engine as E
     from tiledarray_amd import workloads as W
 
@@ -344,158 +434,229 @@ def bench_ladder(args):
     E.make_sparse(ctx, T, thr)
     E.make_sparse(ctx, V, thr)
     sizes = {id(X): E._sizes_vector(X.trange, X.pool.device) for X in (T, V)}
-    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", threshold=thr,
-                      precision=args.precision, tile=fixed_tile(args),
-                      fuse_permute={"auto": None, "copy": False, "map": True}[args.v_permute])
-    # the permuted V's shape follows the source's; refresh it each step
-    perm_norms = c.B.shape_norms
-
-    inv = [int(i) for i in np.argsort(list(c.plan.right_perm[:4]))]
     maxabs = {id(X): torch.zeros(X.trange.ntiles, dtype=torch.float64, device=X.pool.device)
               for X in (T, V)}
     perm_max = torch.zeros(V.trange.ntiles, dtype=torch.float64, device=V.pool.device)
 
-    def step(events):
-        for X in (T, V):
-            E.tile_norms(ctx, X, out=X.shape_norms,
-                         maxabs=maxabs[id(X)] if c.nslices else None)
-            E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
-        c.permute_step()
-        perm_norms.copy_(V.shape_norms.view(V.trange.tiles_shape).permute(*inv).reshape(-1))
-        if c.nslices:
-            perm_max.copy_(maxabs[id(V)].view(V.trange.tiles_shape).permute(*inv).reshape(-1))
-            c.split_and_plan(maxabs=(maxabs[id(T)], perm_max))
-        else:
-            c.shape_step()
-            c.worklist_step()
-        if events is not None:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            c.gemm_step()
-            e1.record()
-            events.append((e0, e1))
-        else:
-            c.gemm_step()
+    def measure(precision):
+        c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", threshold=thr,
+                          precision=precision, tile=fixed_tile(args),
+                          fuse_permute={"auto": None, "copy": False,
+                                        "map": True}[args.v_permute])
+        # the permuted V's shape follows the source's; refresh it each step
+        perm_norms = c.B.shape_norms
+        inv = [int(i) for i in np.argsort(list(c.plan.right_perm[:4]))]
 
-    used_tile = (autotune(args, c, step, reps=1)
-                 if args.precision == "f64" or args.precision.startswith(("bf16", "i8"))
-                 else None)
-    ms, gemm_ms = time_steps(c, step, args.steps, args.warmup)
+        def step(events):
+            for X in (T, V):
+                E.tile_norms(ctx, X, out=X.shape_norms,
+                             maxabs=maxabs[id(X)] if c.nslices else None)
+                E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
+            c.permute_step()
+            perm_norms.copy_(V.shape_norms.view(V.trange.tiles_shape).permute(*inv).reshape(-1))
+            if c.nslices:
+                perm_max.copy_(maxabs[id(V)].view(V.trange.tiles_shape).permute(*inv)
+                               .reshape(-1))
+                c.split_and_plan(maxabs=(maxabs[id(T)], perm_max))
+            else:
+                c.shape_step()
+                c.worklist_step()
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                c.gemm_step()
+                e1.record()
+                events.append((e0, e1))
+            else:
+                c.gemm_step()
+
+        tunable = precision in ("f64", "f32") or precision.startswith(("bf16", "i8"))
+        used_tile = autotune(args, c, step, reps=1) if tunable else None
+        ms, gemm_ms = time_steps(c, step, args.steps, args.warmup)
+        # bandwidth-bound kernels: the V permute (each stored element read
+        # and written once), the norm passes over T and V, the limb split
+        # (a split precision reads V through an index map while splitting
+        # it: no permute pass, no permuted copy)
+        bw = {}
+        if hasattr(c.B, "_perm_plan"):
+            bw["permute"] = (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3))
+        if c.nslices:
+            bw["split_limbs"] = (split_bytes(c, T, V),
+                                 time_kernel(lambda: c.split_step(
+                                     maxabs=(maxabs[id(T)], perm_max)), reps=3))
+        return c, ms, gemm_ms, used_tile, bw
+
+    main_prec = args.precision
+    c, ms, gemm_ms, used_tile, bw = measure(main_prec)
     cnt = c.counters()
     flops = cnt["flops"]
-    peak = (PEAK_FP64_TFLOPS if args.precision == "f64"
-            else PEAK_INT8_TOPS if args.precision.startswith("i8") else PEAK_BF16_TFLOPS)
+    peak = (PEAK_FP64_TFLOPS if main_prec == "f64"
+            else PEAK_INT8_TOPS if main_prec.startswith("i8") else PEAK_BF16_TFLOPS)
+    achieved = flops / (gemm_ms * 1e-3) / 1e12
     out = base_line(args, flops / (ms * 1e-3) / 1e12, ms,
-                    "f64" if args.precision == "f64" else args.precision,
+                    "f64" if main_prec == "f64" else main_prec,
                     f"ladder R(a,b,i,j)=T(c,d,i,j)*V(a,c,b,d), o={args.o}, v={args.v}, "
                     f"v tiles U[48,96], T~N(0,1e-2), V~N(0,1)exp(-|a-c|/64), threshold 1e-10",
                     {"v_tiles": info["v_tiles"], "result_tiles": cnt["ntasks"],
-                     "macro_tile": used_tile or ([64, 64, 16] if args.precision == "f64"
+                     "macro_tile": used_tile or ([64, 64, 16] if main_prec == "f64"
                                                  else [64, 64, 32]),
                      "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
-                     "precision": args.precision, "seed": info["seed"],
+                     "precision": main_prec, "seed": info["seed"],
                      "v_permute": "copy" if hasattr(c.B, "_perm_plan") else "index map"},
-                    {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),
-                     "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,
-                     "avg_launch_ms": round(gemm_ms, 4)})
-    # bandwidth-bound kernels: the V permute (each stored element read and
-    # written once), the norm passes over T and V, the limb split
-    # (a split precision reads V through an index map while splitting it:
-    # no permute pass, no permuted copy)
-    bw = {"tile_norms": (stored_bytes(T) + stored_bytes(V),
-                         time_kernel(lambda: [E.tile_norms(ctx, X, out=X.shape_norms)
-                                              for X in (T, V)], reps=3))}
-    if hasattr(c.B, "_perm_plan"):
-        bw["permute"] = (2 * stored_bytes(V), time_kernel(c.permute_step, reps=3))
-    elif not c.nslices:
+                    {"bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": None, "avg_launch_ms": round(gemm_ms, 4)})
+    if main_prec == "f64" and not hasattr(c.B, "_perm_plan"):
         # the permute kernel on V outside the step (into a scratch copy)
         Y = E.permute_array(ctx, V, list(c.plan.right_perm[:4]))
         bw["permute_standalone"] = (2 * stored_bytes(V),
                                     time_kernel(lambda: Y._perm_plan.run(V.pool, Y.pool), reps=3))
         del Y
-    if c.nslices:
-        # FP64 read, S BF16 (2 B) or INT8 (1 B) limbs written, for both operands
-        nel = (stored_bytes(T) + stored_bytes(c.B)) // 8
-        limb_bytes = 1 if args.precision.startswith("i8") else 2
-        bw["split_limbs"] = (nel * (8 + limb_bytes * c.nslices),
-                             time_kernel(lambda: c.split_step(maxabs=(maxabs[id(T)], perm_max)),
-                                         reps=3))
+    bw["tile_norms"] = (stored_bytes(T) + stored_bytes(V),
+                        time_kernel(lambda: [E.tile_norms(ctx, X, out=X.shape_norms)
+                                             for X in (T, V)], reps=3))
     out["bandwidth"] = bandwidth(bw)
+    sample = None
     if not args.no_cpu:
-        out["cpu_baseline"], out["rel_error"] = ladder_cpu_sample(args, c, T, V)
+        sample = LadderSample(args, c, T, V)
+        out["cpu_baseline"] = sample.cpu_baseline()
+        out["rel_error"] = sample.errors(c)
+    if not args.no_split and args.ladder_split:
+        # the split precisions need the device memory the main run holds
+        # (at o=48, v=320 a permuted V alone is 84 GB)
+        del c
+        torch.cuda.empty_cache()
+        split = {}
+        for prec in args.ladder_split.split(","):
+            cs, ms_s, gemm_s, tile_s, bw_s = measure(prec)
+            entry = {"value": round(flops / (ms_s * 1e-3) / 1e12, 3),
+                     "ms_per_step": round(ms_s, 4),
+                     "gemm_tflops": round(flops / (gemm_s * 1e-3) / 1e12, 3),
+                     "gemm_ms": round(gemm_s, 4),
+                     "speedup_vs_main": round(ms / ms_s, 3),
+                     "macro_tile": tile_s,
+                     "v_permute": "copy" if hasattr(cs.B, "_perm_plan") else "index map",
+                     "roofline": split_roofline(prec, flops, gemm_s),
+                     "bandwidth": bandwidth(bw_s)}
+            assert cs.counters()["npairs"] == cnt["npairs"]
+            if sample is not None:
+                e = sample.errors(cs)
+                entry["rel_err_frobenius"] = e["frobenius_sample"]
+                entry["rel_err_max"] = e["max_sample"]
+                entry["error_against"] = "oracle, " + e["sample"]
+            split[prec] = entry
+            del cs
+            torch.cuda.empty_cache()
+        out["split_precision"] = split
     print(json.dumps(out), flush=True)
 
 
-def ladder_cpu_sample(args, c, T, V):
-    """Oracle over a sample of result-tile columns: per tile permute of the V
-    tiles the sample needs (as TA permutes on read), then the tile GEMMs; the
-    sample's flops over its time give the rate.  Also the GPU result's
-    relative error on the sample."""
-    import numpy as np
+class LadderSample:
+    """The oracle over a sample of whole result tile columns R(a,b,.,.) of
+    the ladder (the (a,b) tiles of smallest area, as many columns as keep the
+    host's threads busy, one task per result tile): per tile permute of the V
+    tiles the sample needs (as TA permutes on read), then the tile GEMMs.
+    Gives the CPU rate (median of 3 after 1 warm-up) and the GPU results'
+    errors on the sample."""
 
-    import oracle
+    def __init__(self, args, c, T, V):
+        import numpy as np
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    oracle.set_num_threads(threads)
-    Th = T.pool.cpu().numpy()
-    rp = [int(x) for x in c.plan.right_perm[:4]]
-    Nt, Kt, Mt = c.Nt, c.Kt, c.Mt
-    c_off_gpu = c.c_off.cpu().numpy()
-    gpu_C = c.C.cpu().numpy()
-    # natural result grid (i,j | a,b) = Mt x Nt; take whole columns j
-    cols, flops, secs = [], 0, 0.0
-    Vsrc = V.pool  # device; tiles copied per use
-    vext = V.trange.tile_extents_table()
-    num = den = 0.0
-    for j in range(Nt):
-        if secs > 15.0:
-            break
-        t0 = time.perf_counter()
-        # B tiles of column j in GEMM layout: (k = (c,d), n = (a,b))
-        b_pool, b_off, pos = [], np.full(Kt * Nt, -1, np.int64), 0
-        for k in range(Kt):
-            bt = k * Nt + j  # ordinal in the permuted grid (c,d,a,b)
-            # source ordinal in V's (a,c,b,d) grid
-            idx_p = np.unravel_index(bt, c.B.trange.tiles_shape)
-            src_idx = [idx_p[rp[d]] for d in range(4)]
-            st = int(np.ravel_multi_index(src_idx, V.trange.tiles_shape))
-            off = V.offsets[st]
-            if off < 0:
-                continue
-            e = vext[st]
-            tile = Vsrc[off:off + int(np.prod(e))].cpu().numpy().reshape([int(x) for x in e])
-            pt = oracle.permute(tile, rp).reshape(-1)
-            b_off[bt] = pos
-            b_pool.append(pt)
-            pos += pt.size
-        Bh = np.concatenate(b_pool) if b_pool else np.zeros(1)
-        c_off = np.full(Mt * Nt, -1, np.int64)
+        import oracle
+        self.threads = cpu_threads(args)
+        oracle.set_num_threads(self.threads)
+        self.Th = T.pool.cpu().numpy()
+        self.rp = rp = [int(x) for x in c.plan.right_perm[:4]]
+        self.Mt, self.Nt, self.Kt = Mt, Nt, Kt = c.Mt, c.Nt, c.Kt
+        # the fused structure only (the contraction itself is not kept, so
+        # its device memory can be released)
+        self.ops = (c.op_a, c.op_b)
+        self.ext = (c.mext_h, c.next_h, c.kext_h)
+        ncols = max(1, min(Nt, self.threads // max(1, Mt)))
+        self.cols = cols = sorted(int(x) for x in np.argsort(c.next_h, kind="stable")[:ncols])
+        c_off_gpu = c.c_off.cpu().numpy()
+        vext = V.trange.tile_extents_table()
+        b_pool, self.b_off, pos = [], np.full(Kt * Nt, -1, np.int64), 0
+        for j in cols:
+            for k in range(Kt):
+                bt = k * Nt + j  # ordinal in the permuted grid (c,d,a,b)
+                idx_p = np.unravel_index(bt, c.B.trange.tiles_shape)
+                st = int(np.ravel_multi_index([idx_p[rp[d]] for d in range(4)],
+                                              V.trange.tiles_shape))
+                off = V.offsets[st]
+                if off < 0:
+                    continue
+                e = [int(x) for x in vext[st]]
+                # the stored (unpermuted) tile; the oracle permutes it in the
+                # timed region, as TA permutes on read
+                b_pool.append((bt, V.pool[off:off + int(np.prod(e))].cpu().numpy().reshape(e)))
+        self.src_tiles = b_pool
+        self.c_off = np.full(Mt * Nt, -1, np.int64)
         vol = 0
-        for i in range(Mt):
-            if c_off_gpu[i * Nt + j] >= 0:
-                c_off[i * Nt + j] = vol
-                vol += int(c.mext_h[i] * c.next_h[j])
-        ref, npairs = oracle.contract(c.op_a, c.op_b, c.mext_h, c.next_h, c.kext_h, Th,
-                                      T.offsets, Bh, b_off, c_off, max(1, vol))
-        secs += time.perf_counter() - t0
-        for i in range(Mt):
-            if c_off[i * Nt + j] < 0:
-                continue
-            m, n = int(c.mext_h[i]), int(c.next_h[j])
-            r = ref[c_off[i * Nt + j]:c_off[i * Nt + j] + m * n].reshape(m, n)
-            og = c_off_gpu[i * Nt + j]
-            g = gpu_C[og:og + m * n].reshape(n, m).T  # stored transposed (a,b,i,j)
-            num += float(np.sum((g - r) ** 2))
-            den += float(np.sum(r * r))
-            flops += 2 * m * n * int(sum(c.kext_h[k] for k in range(Kt) if b_off[k * Nt + j] >= 0))
-        cols.append(j)
-    cpu = {"value": round(flops / secs / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
-           "kind": "port", "blas": oracle.blas_name(), "seconds": round(secs, 3),
-           "sample": f"{len(cols)} of {Nt} result tile columns (per-tile V permute + tile GEMMs)"}
-    return cpu, {"frobenius_sample": (num / den) ** 0.5 if den else 0.0}
+        for j in cols:
+            for i in range(Mt):
+                if c_off_gpu[i * Nt + j] >= 0:
+                    self.c_off[i * Nt + j] = vol
+                    vol += int(c.mext_h[i] * c.next_h[j])
+        self.vol = max(1, vol)
+        self.a_off = np.where(T.shape_norms.cpu().numpy() >= T.shape_threshold, T.offsets, -1)
+        self.seconds, self.ref = median_time(self._run, reps=3, warmup=1)
+        self.flops = 0
+        for j in cols:
+            kk = int(sum(c.kext_h[k] for k in range(Kt) if self.b_off[k * Nt + j] >= 0))
+            for i in range(Mt):
+                if self.c_off[i * Nt + j] >= 0:
+                    self.flops += 2 * int(c.mext_h[i]) * int(c.next_h[j]) * kk
+
+    def _run(self):
+        import numpy as np
+
+        import oracle
+        parts, pos = [], 0
+        for bt, tile in self.src_tiles:
+            pt = oracle.permute(tile, self.rp).reshape(-1)
+            self.b_off[bt] = pos
+            parts.append(pt)
+            pos += pt.size
+        Bh = np.concatenate(parts) if parts else np.zeros(1)
+        ref, _ = oracle.contract(*self.ops, *self.ext, self.Th, self.a_off, Bh, self.b_off,
+                                 self.c_off, self.vol)
+        return ref
+
+    def cpu_baseline(self):
+        import oracle
+        return {"value": round(self.flops / self.seconds / 1e12, 4), "unit": "TFLOP/s",
+                "cores": min(self.threads, len(self.cols) * self.Mt), "kind": "port",
+                "blas": oracle.blas_name(), "seconds": round(self.seconds, 3),
+                "host": host_cpu(),
+                "sample": f"{len(self.cols)} of {self.Nt} result tile columns (per-tile V "
+                          f"permute + tile GEMMs, {self.flops / 1e12:.2f} TFLOP), median of 3 "
+                          f"after 1 warm-up"}
+
+    def errors(self, c):
+        """Relative Frobenius and max errors of contraction c's result over
+        the sample."""
+        import numpy as np
+        off = c.c_off.cpu().numpy()
+        num = den = mx = rmax = 0.0
+        for j in self.cols:
+            for i in range(self.Mt):
+                o = self.c_off[i * self.Nt + j]
+                if o < 0:
+                    continue
+                m, n = int(c.mext_h[i]), int(c.next_h[j])
+                r = self.ref[o:o + m * n].reshape(m, n)
+                og = int(off[i * self.Nt + j])
+                g = c.C[og:og + m * n].cpu().numpy().reshape(n, m).T  # stored (a,b,i,j)
+                num += float(np.sum((g - r) ** 2))
+                den += float(np.sum(r * r))
+                mx = max(mx, float(np.abs(g - r).max()))
+                rmax = max(rmax, float(np.abs(r).max()))
+        return {"frobenius_sample": (num / den) ** 0.5 if den else 0.0,
+                "max_sample": mx / rmax if rmax else 0.0,
+                "sample": f"{len(self.cols)} result tile columns"}
 
 
 def bench_tot(args):
@@ -561,14 +722,16 @@ def bench_tot(args):
           int(np.sum(np.where(w["b_off"] >= 0, c.b_plan.volumes, 0)))) * el
     out["bandwidth"] = bandwidth({"screening": (nb, time_kernel(c.shape_step))})
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         oracle.set_num_threads(threads)
         Ah, Bh = w["A"].cpu().numpy(), w["B"].cpu().numpy()
         c_off = c.c_off.cpu().numpy()
         gotC = c.C.cpu().numpy()
-        t0 = time.perf_counter()
-        parts = []
-        for b in range(args.nbatch):
+
+        def run():
+            return [one(b) for b in range(args.nbatch)]
+
+        def one(b):
             sl = slice(b * g * g, (b + 1) * g * g)
             co = c_off[sl]
             base = int(co[co >= 0].min()) if (co >= 0).any() else 0
@@ -577,8 +740,9 @@ def bench_tot(args):
             local = np.where(co >= 0, co - base, -1)
             part, _ = oracle.contract(0, 0, w["a_dims"][b], w["b_dims"][b], w["c_dims"][b], Ah,
                                       w["a_off"][sl], Bh, w["b_off"][sl], local, max(1, span))
-            parts.append((base, part))
-        secs = time.perf_counter() - t0
+            return base, part
+
+        secs, parts = median_time(run, reps=3, warmup=1)
         num = den = 0.0
         for b in range(args.nbatch):
             base, part = parts[b]
@@ -591,10 +755,35 @@ def bench_tot(args):
                     den += float(np.sum(r * r))
         out["cpu_baseline"] = {"value": round(flops / secs / 1e12, 4), "unit": "TFLOP/s",
                                "cores": threads, "kind": "port", "blas": oracle.blas_name(),
-                               "seconds": round(secs, 3),
-                               "sample": "all instances, tile GEMMs only (shapes not timed)"}
+                               "seconds": round(secs, 3), "host": host_cpu(),
+                               "sample": "all instances, tile GEMMs only (shapes not timed), "
+                                         "median of 3 after 1 warm-up"}
         out["rel_error"] = {"frobenius": (num / den) ** 0.5 if den else 0.0}
     print(json.dumps(out), flush=True)
+
+
+def split_bytes(c, A, B):
+    """Algorithmic bytes of one limb split of both operands: the stored FP64
+    elements read, the padded limb planes written."""
+    rd = stored_bytes(A) + stored_bytes(B)
+    wr = sum(int(sp.limbs.numel()) * sp.limbs.element_size() for sp in (c.a_split, c.b_split))
+    return rd + wr
+
+
+def near_threshold(ctx, c):
+    """Result-shape estimates within 1e-5 relative of the threshold (the
+    tiles whose presence would hinge on the estimate's last bits), from one
+    unscreened estimate outside the timed steps."""
+    import ctypes
+    import torch
+    from tiledarray_amd import engine as E
+    est = torch.empty_like(c.c_norms)
+    E.call("tamx_shape_gemm_f32", ctx.ptr, c.A.shape_norms.data_ptr(),
+           c.B.shape_norms.data_ptr(), c.op_a, c.op_b, c.Mt, c.Nt, c.Kt, 1,
+           c.k_sizes.data_ptr(), ctypes.c_float(abs(c.alpha)), ctypes.c_float(0.0),
+           est.data_ptr(), None, ctx.stream())
+    e = est.cpu().numpy().astype("float64")
+    return int(((e - c.threshold) ** 2 <= (1e-5 * c.threshold) ** 2).sum())
 
 
 def used_mask(c):
@@ -616,7 +805,7 @@ def cpu_leg(args, c, A, B, thr, sparse, gpu_flops):
 
     import oracle
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
     oracle.set_num_threads(threads)
     Ah, Bh = A.pool.cpu().numpy(), B.pool.cpu().numpy()
     vols = A.trange.tile_volumes()
@@ -624,9 +813,8 @@ def cpu_leg(args, c, A, B, thr, sparse, gpu_flops):
     nt = A.trange.tiles_shape[0]
     c_off_gpu = c.c_off.cpu().numpy()
     gpu_C = c.C.cpu().numpy()
-    times = []
-    for rep in range(2):
-        t0 = time.perf_counter()
+
+    def run():
         if sparse:
             na = oracle.tile_norms(Ah, A.offsets, vols).reshape(nt, nt)
             nb = oracle.tile_norms(Bh, B.offsets, vols).reshape(nt, nt)
@@ -645,8 +833,9 @@ def cpu_leg(args, c, A, B, thr, sparse, gpu_flops):
         c_off[present] = np.concatenate([[0], np.cumsum(vol_c[present])])[:-1]
         ref, npairs = oracle.contract(0, 0, mext, c.next_h, c.kext_h, Ah, a_off, Bh, b_off, c_off,
                                       int(vol_c[present].sum()))
-        times.append(time.perf_counter() - t0)
-    t = min(times)
+        return present, vol_c, c_off, ref, npairs
+
+    t, (present, vol_c, c_off, ref, npairs) = median_time(run, reps=3, warmup=1)
     pattern_equal = bool(np.array_equal(present, c_off_gpu >= 0))
     # relative Frobenius error of the GPU result over the common pattern
     num = den = 0.0
@@ -662,9 +851,11 @@ def cpu_leg(args, c, A, B, thr, sparse, gpu_flops):
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": "full workload (norms, scaling, shape estimate, all tile GEMMs), best of 2",
+        "sample": "full workload (norms, scaling, shape estimate, all tile GEMMs), "
+                  "median of 3 after 1 warm-up",
         "seconds": round(t, 3),
         "blas": oracle.blas_name(),
+        "host": host_cpu(),
         "pattern_equal": pattern_equal,
         "tile_pairs": npairs,
     }

@@ -390,335 +390,6 @@ def test_split_i8_limbs_ties(ctx, is_right, op):
     assert t0.sum() >= 2 * j.size and np.all(L[0, :m, :k][t0] % 2 == 0)
 
 
-@pytest.mark.parametrize("nslices", [2, 3, 5, 6])
-@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_split_integer_exact(ctx, nslices, op_a, op_b):
-    rng = np.random.default_rng(300 + nslices * 4 + op_a * 2 + op_b)
-    trm = TiledRange1([0, 40, 130, 200])
-    trk = TiledRange1([0, 17, 64, 150, 151, 300])
-    trn = TiledRange1([0, 96, 97, 180])
-    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
-    la = "k,m" if op_a else "m,k"
-    lb = "n,k" if op_b else "k,n"
-    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=f"bf16x{nslices}")
-    c.run()
-    torch.cuda.synchronize()
-    Cref, npairs, c_off = oracle_contract(c, A, B)
-    mask = c_off_mask(c_off, c)
-    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
-
-
-def test_norms_with_max_and_premax_split(ctx):
-    # the fused norm + max pass gives the same norms bit for bit and each
-    # tile's exact max |a|; the split fed with those maxima writes the same
-    # limbs as the two-pass split, and the contraction is unchanged
-    A, B, _ = W.blocksparse_gemm(n=1200, wlo=24, whi=96, seed=8)
-    E.make_sparse(ctx, A, 1e-8)
-    E.make_sparse(ctx, B, 1e-8)
-    for X in (A, B):
-        plain = E.tile_norms(ctx, X)
-        mx = torch.full((X.trange.ntiles,), -1.0, dtype=torch.float64, device="cuda")
-        fused = E.tile_norms(ctx, X, maxabs=mx)
-        torch.cuda.synchronize()
-        assert torch.equal(plain, fused)
-        ref = np.zeros(X.trange.ntiles)
-        for t in range(X.trange.ntiles):
-            if X.offsets[t] >= 0:
-                ref[t] = np.abs(X.tile(t).cpu().numpy()).max()
-        np.testing.assert_array_equal(mx.cpu().numpy(), ref)
-        X._mx = mx
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="bf16x4")
-    c.run()
-    torch.cuda.synchronize()
-    limbs = [c.a_split.limbs.clone(), c.b_split.limbs.clone()]
-    C0 = c.C.clone()
-    for sp in (c.a_split, c.b_split):
-        sp.limbs.zero_()
-    c.split_step(maxabs=(A._mx, B._mx))
-    c.gemm_step()
-    torch.cuda.synchronize()
-    assert torch.equal(c.a_split.limbs, limbs[0]) and torch.equal(c.b_split.limbs, limbs[1])
-    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
-    assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
-
-
-@pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
-@pytest.mark.parametrize("precision", ["bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
-@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_split_wide_tiles_integer_exact(ctx, tile, precision, op_a, op_b):
-    # the wide macro tiles (one wave per SIMD) on tiles wider and narrower
-    # than one macro tile, ragged in both directions
-    nslices = int(precision.split("x")[-1])
-    rng = np.random.default_rng(900 + nslices * 4 + op_a * 2 + op_b + 50 * ("i8" in precision))
-    trm = TiledRange1([0, 40, 170, 300, 301])
-    trk = TiledRange1([0, 17, 64, 150, 151, 300])
-    trn = TiledRange1([0, 96, 97, 250])
-    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
-    la = "k,m" if op_a else "m,k"
-    lb = "n,k" if op_b else "k,n"
-    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=precision, tile=tile)
-    c.run()
-    torch.cuda.synchronize()
-    Cref, npairs, c_off = oracle_contract(c, A, B)
-    mask = c_off_mask(c_off, c)
-    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
-
-
-@pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
-@pytest.mark.parametrize("precision", ["bf16x4", "i8x4"])
-def test_split_wide_tiles_match_default(ctx, tile, precision):
-    # same limbs, same pair order: the wide kernel's result equals the
-    # default kernel's bit for bit on random data
-    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=21)
-    E.make_sparse(ctx, A, 1e-8)
-    E.make_sparse(ctx, B, 1e-8)
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision=precision)
-    c.run()
-    torch.cuda.synchronize()
-    C0 = c.C.clone()
-    c.set_tile(tile)
-    c.C.zero_()
-    c.gemm_step()
-    torch.cuda.synchronize()
-    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
-    assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
-
-
-def test_split_sigma_is_tile_exponent(ctx):
-    rng = np.random.default_rng(5)
-    tr = TiledRange1([0, 20, 45])
-    A, B = matrix_ops_arrays(rng, tr, tr, tr, 0, 0, torch.float64, False)
-    A.pool.mul_(1e-3)
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision="bf16x3")
-    c.run()
-    torch.cuda.synchronize()
-    sa, sb = c.sigmas()
-    for X, sig in ((A, sa), (B, sb)):
-        for t in range(X.trange.ntiles):
-            tile = X.tile(t).cpu().numpy()
-            m = np.abs(tile).max()
-            assert 2.0 ** (sig[t] - 1) <= m < 2.0 ** sig[t]
-
-
-@pytest.mark.parametrize("nslices", [2, 3, 4, 5, 6])
-def test_split_blocksparse_error_bound(ctx, nslices):
-    thr = 1e-8
-    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=21)
-    E.make_sparse(ctx, A, thr)
-    E.make_sparse(ctx, B, thr)
-    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=thr, precision=f"bf16x{nslices}")
-    c.run()
-    torch.cuda.synchronize()
-    sa, sb = A.shape_norms.cpu().numpy(), B.shape_norms.cpu().numpy()
-    Cref, npairs, c_off = oracle_contract(c, A, B, (A.offsets >= 0) & (sa >= thr),
-                                          (B.offsets >= 0) & (sb >= thr))
-    mask = c_off_mask(c_off, c)
-    err = rel_fro(c.C.cpu().numpy()[mask], Cref[mask])
-    assert err <= SPLIT_TOL[nslices], (nslices, err)
-
-
-@pytest.mark.parametrize("nslices", [3, 4, 5])
-def test_split_ladder_error_bound(ctx, nslices):
-    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=3)
-    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision=f"bf16x{nslices}")
-    c.run()
-    R = c.result()
-    ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
-    err = rel_fro(R.to_dense(), ref)
-    assert err <= LADDER_TOL[nslices], (nslices, err)
-
-
-@pytest.mark.parametrize("nslices", [2, 3, 4, 5])
-@pytest.mark.parametrize("is_right,op", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_split_limbs_closed_form(ctx, nslices, is_right, op):
-    # the limbs of one tile against the expansion's definition:
-    #   L_s 2^(9s) is an integer l_s, |l_0| <= 256, |l_s| <= 256,
-    #   |a - 2^(sigma-8) sum_s L_s| <= 2^(sigma - 9S), padding is zero
-    import ctypes
-    from tiledarray_amd import _lib
-    rows, cols = 37, 45  # stored (row-major) tile
-    rng = np.random.default_rng(700 + nslices)
-    a = rng.uniform(-1, 1, (rows, cols)) * 3.7 * 2.0 ** rng.integers(-3, 4, (rows, cols))
-    pool = torch.as_tensor(a.reshape(-1), device="cuda")
-    plan = ctypes.c_void_p()
-    off = np.zeros(1, np.int64)
-    r32 = np.array([rows], np.int32)
-    c32 = np.array([cols], np.int32)
-    E.call("tamx_split_plan_create", ctx.ptr, nslices, is_right, op, 1, off.ctypes.data,
-           r32.ctypes.data, c32.ctypes.data, ctypes.byref(plan))
-    try:
-        n = _lib.lib().tamx_split_plan_limb_elems(plan)
-        limbs = torch.zeros(n, dtype=torch.bfloat16, device="cuda")
-        E.call("tamx_split_f64", ctx.ptr, plan, pool.data_ptr(), limbs.data_ptr(), ctx.stream())
-        torch.cuda.synchronize()
-        sig = np.zeros(1, np.int32)
-        E.call("tamx_split_plan_sigmas", plan, sig.ctypes.data)
-    finally:
-        _lib.lib().tamx_split_plan_destroy(plan)
-    sigma = int(sig[0])
-    assert 2.0 ** (sigma - 1) <= np.abs(a).max() < 2.0 ** sigma
-    # logical limb layout: op(A) as m x K (A) or op(B)^T as n x K (B)
-    trans = (op == 1) if not is_right else (op == 0)
-    logical = a.T if trans else a
-    m, k = logical.shape
-    mp, kp = -(-m // 16) * 16, -(-k // 32) * 32
-    L = limbs.float().cpu().numpy().astype(np.float64)[:nslices * mp * kp]
-    L = L.reshape(nslices, mp, kp)
-    for s_ in range(nslices):
-        l = L[s_] * 2.0 ** (9 * s_)
-        assert np.array_equal(l, np.rint(l)), s_
-        assert np.abs(l).max() <= 256, s_
-    assert not L[:, m:, :].any() and not L[:, :, k:].any()
-    rec = 2.0 ** (sigma - 8) * L[:, :m, :k].sum(axis=0)
-    assert np.abs(logical - rec).max() <= 2.0 ** (sigma - 9 * nslices)
-
-
-def test_split_error_drops_nine_bits_per_limb(ctx):
-    # one more limb divides the error by about 2^9 (between 2^7 and 2^11)
-    A, B, _ = W.dense_gemm(n=384, tile=128)
-    errs = {}
-    ref = A.to_dense() @ B.to_dense()
-    for S in (2, 3, 4):
-        c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", precision=f"bf16x{S}")
-        c.run()
-        errs[S] = rel_fro(c.result().to_dense(), ref)
-    for S in (2, 3):
-        ratio = errs[S] / errs[S + 1]
-        assert 2.0 ** 7 <= ratio <= 2.0 ** 11, (S, errs)
-
-
-@pytest.mark.parametrize("nslices", [3, 4])
-def test_split_fused_permute_matches_permuted_copy(ctx, nslices):
-    # V stored (a,c,b,d) is split straight from its stored layout through an
-    # index map; the limbs and the result equal those from a permuted copy
-    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=4)
-    cf = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d",
-                       precision=f"bf16x{nslices}")
-    cp = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d",
-                       precision=f"bf16x{nslices}", fuse_permute=False)
-    assert hasattr(cf.B, "_view_src") and hasattr(cp.B, "_perm_plan")
-    cf.run()
-    cp.run()
-    torch.cuda.synchronize()
-    assert torch.equal(cf.b_split.limbs, cp.b_split.limbs)
-    assert torch.equal(cf.a_split.limbs, cp.a_split.limbs)
-    np.testing.assert_array_equal(cf.result().to_dense(), cp.result().to_dense())
-
-
-@pytest.mark.parametrize("annot", ["c,a,d,b", "d,c,b,a", "b,d,a,c"])
-def test_split_fused_permute_other_orders(ctx, annot):
-    # other general permutations of the right operand (the stride-1 source
-    # mode lands in the limb rows or in k), integer data: exact
-    rng = np.random.default_rng(11)
-    tv = TiledRange1([0, 5, 12, 20])
-    to = TiledRange1([0, 3, 6])
-    T = DistArray.allocate(TiledRange([tv, tv, to, to]), rng.random(36) < 0.9)
-    T.pool.copy_(torch.as_tensor(rng.integers(-8, 9, T.pool.numel()), dtype=torch.float64))
-    V = DistArray.allocate(TiledRange([tv, tv, tv, tv]), rng.random(81) < 0.9)
-    V.pool.copy_(torch.as_tensor(rng.integers(-8, 9, V.pool.numel()), dtype=torch.float64))
-    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, annot, precision="bf16x3")
-    c.run()
-    R = c.result().to_dense()
-    ref = np.einsum("cdij," + annot.replace(",", "") + "->abij", T.to_dense(), V.to_dense())
-    np.testing.assert_array_equal(R, ref)
-    # the same operand on the left
-    c = E.Contraction(ctx, "i,j,a,b", V, annot, T, "c,d,i,j", precision="bf16x3")
-    c.run()
-    np.testing.assert_array_equal(c.result().to_dense(), ref.transpose(2, 3, 0, 1))
-
-
-@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_f32x2_integer_exact(ctx, op_a, op_b):
-    # small integers split exactly into hi (lo = 0) and their products and
-    # 16-term FP32 sums are exact
-    rng = np.random.default_rng(400 + op_a * 2 + op_b)
-    trm = TiledRange1([0, 40, 130])
-    trk = TiledRange1([0, 17, 64, 150])
-    trn = TiledRange1([0, 96, 97, 180])
-    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True)
-    c = E.Contraction(ctx, "m,n", A, "k,m" if op_a else "m,k", B, "n,k" if op_b else "k,n",
-                      precision="f32x2")
-    c.run()
-    torch.cuda.synchronize()
-    Cref, _, c_off = oracle_contract(c, A, B)
-    mask = c_off_mask(c_off, c)
-    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
-
-
-def test_f32x2_error_bound(ctx):
-    # FP32 accumulation rounds: ~24 bits, bound 1e-6 relative (Frobenius)
-    T, V, _ = W.ladder(o=8, v=40, otiles=2, wlo=9, whi=17, seed=3)
-    c = E.Contraction(ctx, "a,b,i,j", T, "c,d,i,j", V, "a,c,b,d", precision="f32x2")
-    c.run()
-    R = c.result()
-    ref = np.einsum("cdij,acbd->abij", T.to_dense(), V.to_dense())
-    err = rel_fro(R.to_dense(), ref)
-    assert err <= 1e-6, err
-
-
-# ------------------------------------------------------------- INT8 limbs --
-# relative Frobenius error bounds for INT8 limbs of 7 bits: a numpy
-# simulation of the same expansion on U[-1,1] 512^3 products (64-wide tiles)
-# gives 9.5e-7 / 8.3e-9 / 7.1e-11 / 6.0e-13 at 3 / 4 / 5 / 6 limbs (the
-# INT32 levels are exact, so no accumulator floor); the bounds leave ~30x
-I8_TOL = {3: 3e-5, 4: 3e-7, 5: 2e-9, 6: 2e-11}
-
-
-@pytest.mark.parametrize("nslices", [2, 3, 4, 5, 6])
-@pytest.mark.parametrize("is_right,op", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_split_i8_limbs_closed_form(ctx, nslices, is_right, op):
-    # the INT8 limbs of one tile against their definition:
-    #   |l_0| <= 127, |l_s| <= 64 after it,
-    #   |a - 2^(sigma-7) sum_s l_s 2^-7s| <= 2^(sigma - 7S - 1),
-    #   K padded to 64, padding zero
-    import ctypes
-    from tiledarray_amd import _lib
-    rows, cols = 37, 45
-    rng = np.random.default_rng(800 + nslices)
-    a = rng.uniform(-1, 1, (rows, cols)) * 3.7 * 2.0 ** rng.integers(-3, 4, (rows, cols))
-    pool = torch.as_tensor(a.reshape(-1), device="cuda")
-    plan = ctypes.c_void_p()
-    off = np.zeros(1, np.int64)
-    r32 = np.array([rows], np.int32)
-    c32 = np.array([cols], np.int32)
-    E.call("tamx_split_plan_create_typed", ctx.ptr, _lib.LIMB_I8, nslices, is_right, op, 1,
-           off.ctypes.data, r32.ctypes.data, c32.ctypes.data, ctypes.byref(plan))
-    try:
-        assert _lib.lib().tamx_split_plan_limb_type(plan) == _lib.LIMB_I8
-        n = _lib.lib().tamx_split_plan_limb_elems(plan)
-        limbs = torch.full((n,), 99, dtype=torch.uint8, device="cuda")
-        E.call("tamx_split_f64", ctx.ptr, plan, pool.data_ptr(), limbs.data_ptr(), ctx.stream())
-        torch.cuda.synchronize()
-        sig = np.zeros(1, np.int32)
-        E.call("tamx_split_plan_sigmas", plan, sig.ctypes.data)
-    finally:
-        _lib.lib().tamx_split_plan_destroy(plan)
-    sigma = int(sig[0])
-    trans = (op == 1) if not is_right else (op == 0)
-    logical = a.T if trans else a
-    m, k = logical.shape
-    mp, kp = -(-m // 16) * 16, -(-k // 64) * 64
-    L = limbs.cpu().numpy().view(np.int8).astype(np.float64)[:nslices * mp * kp]
-    L = L.reshape(nslices, mp, kp)
-    assert np.abs(L[0]).max() <= 127 and np.abs(L[1:]).max() <= 64
-    assert np.array_equal(L, np.rint(L))
-    assert not L[:, m:, :].any() and not L[:, :, k:].any()
-    # bit for bit: l_s = rint(x_s) (ties to even), x_0 = a 2^(7-sigma),
-    # x_{s+1} = 128 (x_s - l_s), every step exact in FP64
-    xs = np.ldexp(logical, 7 - sigma)
-    for s in range(nslices):
-        ls = np.rint(xs)
-        assert np.array_equal(L[s, :m, :k], ls), s
-        xs = (xs - ls) * 128.0
-    w = 2.0 ** (-7 * np.arange(nslices))
-    rec = 2.0 ** (sigma - 7) * np.tensordot(w, L[:, :m, :k], axes=1)
-    assert np.abs(logical - rec).max() <= 2.0 ** (sigma - 7 * nslices - 1)
-    # sigma: the tile max in [2^(sigma-1), 2^sigma), or just below 2^(sigma-1)
-    # when it would round to 128 in the first limb
-    mx = np.abs(a).max()
-    assert mx < 2.0 ** sigma and (mx >= 2.0 ** (sigma - 1) or mx * 2.0 ** (8 - sigma) > 127)
-
-
 @pytest.mark.parametrize("precision", ["f64", "bf16x3", "i8x4"])
 @pytest.mark.parametrize("empty", ["A", "B", "both"])
 @pytest.mark.parametrize("sparse", [True, False])

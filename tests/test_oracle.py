@@ -237,11 +237,14 @@ def _dense_of(pool, offs, ext_r, ext_c, trans=False):
     return out
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_contract_integer_exact_vs_dense_product(op_a, op_b):
+def test_contract_integer_exact_vs_dense_product(op_a, op_b, dtype):
     # integer-valued inputs make every partial sum exact, so the block-sparse
     # one-rank Summa must equal the dense product exactly on the non-zero
-    # result tiles (tests/dist_eval_contraction_eval.cpp:375-472 pattern)
+    # result tiles (tests/dist_eval_contraction_eval.cpp:375-472 pattern);
+    # FP32 pools run the float restatement (ora_contract_f32), whose sums of
+    # at most 40 products of integers below 2^6 are exact in float too
     rng = np.random.default_rng(20 + op_a * 2 + op_b)
     em, ek, en = rng.integers(1, 9, 4), rng.integers(1, 9, 5), rng.integers(1, 9, 3)
     am = rng.random((4, 5)) < 0.6
@@ -265,7 +268,9 @@ def test_contract_integer_exact_vs_dense_product(op_a, op_b):
         if cmask.reshape(-1)[t]:
             coff[t] = pos
             pos += em[t // 3] * en[t % 3]
+    A, B = A.astype(dtype), B.astype(dtype)
     C, npairs = oracle.contract(op_a, op_b, em, en, ek, A, aoff, B, boff, coff, pos, alpha=2.0)
+    assert C.dtype == dtype
     ref = 2.0 * Ad @ Bd
     Cd = _dense_of(C, coff, em, en)
     bm_r, bn_r = np.concatenate([[0], np.cumsum(em)]), np.concatenate([[0], np.cumsum(en)])

@@ -776,7 +776,7 @@ class BatchedContraction:
     BF16 limbs, each cell one tile of the split plan)."""
 
     def __init__(self, ctx, nbatch, Mt, Nt, Kt, mdims, ndims, kdims, A, a_off, B, b_off,
-                 threshold=FLOAT_EPS, alpha=1.0, tile=None, precision=None):
+                 threshold=FLOAT_EPS, alpha=1.0, tile=None, precision=None, cell_shapes=True):
         self.ctx = ctx
         dev = A.device
         self.nbatch, self.Mt, self.Nt, self.Kt = int(nbatch), int(Mt), int(Nt), int(Kt)
@@ -792,19 +792,24 @@ class BatchedContraction:
         self.b_off_h = np.ascontiguousarray(b_off, np.int64)
         self.d_a_off = torch.as_tensor(self.a_off_h, device=dev)
         self.d_b_off = torch.as_tensor(self.b_off_h, device=dev)
-        a_vol = (md[:, :, None] * kd[:, None, :]).reshape(-1)
-        b_vol = (kd[:, :, None] * nd[:, None, :]).reshape(-1)
-        self.a_plan = NormPlan(ctx, self.a_off_h, a_vol)
-        self.b_plan = NormPlan(ctx, self.b_off_h, b_vol)
         self.a_norms = torch.zeros(nbatch * Mt * Kt, dtype=torch.float32, device=dev)
         self.b_norms = torch.zeros(nbatch * Kt * Nt, dtype=torch.float32, device=dev)
         self.c_norms = torch.zeros(nbatch * Mt * Nt, dtype=torch.float32, device=dev)
-        f32 = np.float32
-        self.a_sizes = torch.as_tensor(np.concatenate([md, kd], axis=1).astype(f32).reshape(-1),
-                                       device=dev)
-        self.b_sizes = torch.as_tensor(np.concatenate([kd, nd], axis=1).astype(f32).reshape(-1),
-                                       device=dev)
-        self.k_sizes = torch.as_tensor(kd.astype(f32).reshape(-1), device=dev)
+        # cell_shapes=False: the caller writes a_norms, b_norms and c_norms
+        # itself (FusedContraction's per-tile screening), so the cell norm
+        # plans and size vectors of shape_step are not built
+        self.cell_shapes = bool(cell_shapes)
+        if self.cell_shapes:
+            a_vol = (md[:, :, None] * kd[:, None, :]).reshape(-1)
+            b_vol = (kd[:, :, None] * nd[:, None, :]).reshape(-1)
+            self.a_plan = NormPlan(ctx, self.a_off_h, a_vol)
+            self.b_plan = NormPlan(ctx, self.b_off_h, b_vol)
+            f32 = np.float32
+            self.a_sizes = torch.as_tensor(
+                np.concatenate([md, kd], axis=1).astype(f32).reshape(-1), device=dev)
+            self.b_sizes = torch.as_tensor(
+                np.concatenate([kd, nd], axis=1).astype(f32).reshape(-1), device=dev)
+            self.k_sizes = torch.as_tensor(kd.astype(f32).reshape(-1), device=dev)
         self.d_mext = torch.as_tensor(md.astype(np.int32).reshape(-1), device=dev)
         self.d_next = torch.as_tensor(nd.astype(np.int32).reshape(-1), device=dev)
         self.d_kext = torch.as_tensor(kd.astype(np.int32).reshape(-1), device=dev)
@@ -862,6 +867,8 @@ class BatchedContraction:
 
     def shape_step(self):
         """Cell norms, inverse-volume scaling, batched result estimate."""
+        if not self.cell_shapes:
+            raise RuntimeError("BatchedContraction: built without cell shapes")
         s = self.ctx.stream()
         f = "tamx_tile_norms_f64" if self.dtype == torch.float64 else "tamx_tile_norms_f32"
         call(f, self.ctx.ptr, self.a_plan.ptr, self.A.data_ptr(), self.a_norms.data_ptr(), s)
@@ -993,12 +1000,13 @@ class FusedContraction:
         b_off = np.where(bo[t] >= 0, bo[t] + p[:, None, None] * b_slice[None], -1)
         thr = 0.0 if threshold is None else float(np.float32(threshold))
         nb = max(N, 1)
+        self.tile_shapes = threshold is not None
         self.bc = BatchedContraction(ctx, nb, mt, jt, kt, np.tile(me, nb), np.tile(je, nb),
                                      np.tile(ke, nb), A.pool,
                                      a_off.reshape(-1) if N else np.full(mt * kt, -1),
                                      B.pool, b_off.reshape(-1) if N else np.full(kt * jt, -1),
-                                     threshold=thr, alpha=alpha, precision=precision)
-        self.tile_shapes = threshold is not None
+                                     threshold=thr, alpha=alpha, precision=precision,
+                                     cell_shapes=not self.tile_shapes)
         if self.tile_shapes:
             if A.shape_norms is None or B.shape_norms is None:
                 raise ValueError("FusedContraction: a threshold needs A and B with shapes "
@@ -1022,10 +1030,15 @@ class FusedContraction:
              ctypes.c_float(abs(self.alpha)), ctypes.c_float(self.threshold),
              self.est.data_ptr(), None, self.ctx.stream())
         if self.N:
-            t = self.d_elem_tile
-            self.bc.a_norms.copy_(self.A.shape_norms.view(nt, mt, kt)[t].reshape(-1))
-            self.bc.b_norms.copy_(self.B.shape_norms.view(nt, kt, jt)[t].reshape(-1))
-            self.bc.c_norms.copy_(self.est.view(nt, mt, jt)[t].reshape(-1))
+            # gathers into the preallocated per-element buffers (no
+            # temporaries, so the step can be captured in a graph)
+            t, N = self.d_elem_tile, self.N
+            torch.index_select(self.A.shape_norms.view(nt, mt * kt), 0, t,
+                               out=self.bc.a_norms.view(N, mt * kt))
+            torch.index_select(self.B.shape_norms.view(nt, kt * jt), 0, t,
+                               out=self.bc.b_norms.view(N, kt * jt))
+            torch.index_select(self.est.view(nt, mt * jt), 0, t,
+                               out=self.bc.c_norms.view(N, mt * jt))
 
     def run(self):
         if self.tile_shapes:
@@ -1052,6 +1065,12 @@ class FusedContraction:
         R = DistArray.allocate(trange, present.reshape(-1), dtype=self.A.pool.dtype,
                                device=self.A.pool.device)
         R.pool.zero_()
+        if self.tile_shapes:
+            # the screened result carries its shape: the per-tile estimate,
+            # (F, I, J) tiles in row-major order as the result's tile grid
+            # (SparseShape::gemm_batched, sparse_shape.h:1695-1880)
+            R.shape_norms = self.est.clone()
+            R.shape_threshold = self.threshold
         ro = R.offsets.reshape(ntiles_f, mt, jt)
         me, je = self.me, self.je
         n, i, j = np.nonzero(c_off >= 0)
