@@ -8,7 +8,8 @@ oracle bit for bit; U[-1,1] inputs reach relative Frobenius error <= 1e-13.
 1024 independent tensor-of-tensor products C_n(i,j;a,b) = A_n(i,k;a,c) *
 B_n(k,j;c,b) (8x8 outer grids, half the cells present, inner extents
 U[16,96]), FP32 and FP64, in one persistent grouped launch captured in a
-hipGraph: shapes and patterns bit-exact per instance; FP64 integer cells
+hipGraph: patterns bit-exact per instance, shapes bit-exact for FP64 cells
+and within float rounding for FP32 cells; FP64 integer cells
 bit-exact, FP64 U[-1,1] cells <= 1e-13 and FP32 cells <= 1e-5 relative
 Frobenius error per instance; every replay of the graph equals the eager
 result bit for bit.
@@ -88,8 +89,14 @@ def test_tot_1024_graph_against_oracle(ctx, dtype, integer, tol):
     graph.replay()
     torch.cuda.synchronize()
     Cref, est, c_off = oracle_batch(w, c)
-    # shapes and the result pattern, bit for bit
-    np.testing.assert_array_equal(c.c_norms.cpu().numpy(), est)
+    # the result pattern bit for bit; the shape estimate bit for bit for
+    # FP64 cells (norms summed in double, then cast to float), within float
+    # rounding for FP32 cells (TA sums their squares in float, sequentially,
+    # tensor/tensor.h:3768-3776; the device sums in float in another order)
+    if dtype == torch.float64:
+        np.testing.assert_array_equal(c.c_norms.cpu().numpy(), est)
+    else:
+        np.testing.assert_allclose(c.c_norms.cpu().numpy(), est, rtol=1e-5, atol=0)
     np.testing.assert_array_equal(c_off >= 0, est >= c.threshold)
     cnt = c.counters()
     assert cnt["ntasks"] == int((c_off >= 0).sum()) > 1024 * 8

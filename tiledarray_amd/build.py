@@ -70,24 +70,5 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     return LIB
 
 
-def build_cpp_tests(verbose: bool = False) -> str:
-    """Compiles the C++ host-interface tests (tests/cpp/*.cpp) against
-    include/tiledarray_amd.hpp and libtamx.so."""
-    src = os.path.join(ROOT, "tests", "cpp", "test_contract.cpp")
-    out = os.path.join(ROOT, "tests", "cpp", "test_contract.bin")
-    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-o", out + ".tmp", src, "-L", HERE, "-ltamx",
-           "-Wl,-rpath,$ORIGIN/../../tiledarray_amd"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("C++ test build failed:\n" + r.stdout + r.stderr)
-    os.replace(out + ".tmp", out)
-    return out
-
-
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
-    print(build_cpp_tests(verbose="-v" in sys.argv))
