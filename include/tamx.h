@@ -191,6 +191,25 @@ int tamx_tile_norms_max_f64(tamx_context* ctx, const tamx_norm_plan* plan,
                             void* stream);
 int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
                         const float* pool, float* norms, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the norms of tamx_tile_norms_f64, bit for bit, without
+ * a plan.  offsets and volumes are the tile table in device memory (int64
+ * [ntiles]; offset < 0: no tile, norm 0); pool_elems is the length of pool,
+ * inside which the tiles lie.  The chunk table and partial sums live in a
+ * static buffer of the library (about 10 MiB per GPU, held until the process
+ * ends), so nothing is allocated.  Limits: ntiles <= TAMX_NOALLOC_MAX_TILES
+ * and pool_elems / 65536 + ntiles <= TAMX_NOALLOC_MAX_CHUNKS, else
+ * TAMX_ERR_CAPACITY before any launch.  A table the device finds invalid (a
+ * present tile with negative volume, or tiles outside the pool) gives NaN
+ * norms.  Every call on one GPU uses the same buffer: calls that overlap on
+ * one GPU (other streams, other threads) race. */
+#define TAMX_NOALLOC_MAX_TILES 262144
+#define TAMX_NOALLOC_MAX_CHUNKS 1048576
+int tamx_tile_norms_f64_noalloc(tamx_context* ctx, const int64_t* offsets,
+                                const int64_t* volumes, int64_t ntiles,
+                                const double* pool, int64_t pool_elems,
+                                float* norms, void* stream);
 
 /* Scales raw tile norms in place by inverse tile volume and screens them.
  * Replaces SparseShape's norm constructor (sparse_shape.h:335-349, scaling

@@ -18,6 +18,7 @@ WL_NTASKS, WL_NPAIRS, WL_CELEMS, WL_STATUS, WL_FLOPS, WL_NCOUNTERS = 0, 1, 2, 3,
 PLAN_MAX_RANK, PLAN_NAME_LEN = 8, 64
 PREC_F64, PREC_F32, PREC_BF16X3, PREC_BF16XS, PREC_F32X2 = 0, 1, 2, 3, 4
 TUNE_FORCE, TUNE_NO_WARMUP = 1, 2
+NOALLOC_MAX_TILES, NOALLOC_MAX_CHUNKS = 262144, 1048576
 LIMB_BF16, LIMB_I8 = 0, 1
 
 
@@ -111,6 +112,7 @@ SIGNATURES = {
     "tamx_norm_plan_destroy": [_vp],
     "tamx_tile_norms_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_tile_norms_f32": [_vp, _vp, _vp, _vp, _vp],
+    "tamx_tile_norms_f64_noalloc": [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp],
     "tamx_shape_scale": [_vp, _vp, _i, _vp, _i64, _vp, _i, _f, _vp, _vp],
     "tamx_shape_gemm_f32": [_vp, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _vp, _f, _f, _vp,
                             _vp, _vp],

@@ -63,6 +63,10 @@ int tamx_context_create(int device, tamx_context** out) {
     delete ctx;
     return TAMX_ERR_UNSUPPORTED;
   }
+  if (int rc = tamx::norm_scratch_address(ctx)) {
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return TAMX_OK;
 }

@@ -18,6 +18,8 @@ struct Context {
   int num_cus = 0;
   int max_shared_per_block = 0;
   hipDeviceProp_t prop{};
+  // device address of the static scratch of tamx_tile_norms_f64_noalloc
+  void* norm_scratch = nullptr;
 };
 
 // Makes the context's device current for the calling thread.
@@ -77,6 +79,8 @@ constexpr int kScanBlocks = 256;  // workgroups of the multi-workgroup scans
 // wl->item_capacity.  Three launches on `s`; reads the task count from the
 // work list's counters, so the sequence is graph-capturable.
 int plan_items(tamx_worklist* wl, int bm, int bn, hipStream_t s);
+// Resolves ctx->norm_scratch on ctx's device (which must be current).
+int norm_scratch_address(tamx_context* ctx);
 }
 
 #define TAMX_CHECK_HIP(expr)                                                   \

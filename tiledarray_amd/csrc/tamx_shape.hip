@@ -10,7 +10,10 @@
 // for bit.  (HIP's __fmul_rn/__fadd_rn are header functions whose operations
 // keep the contractable flag, so they are not used here.)
 #include <algorithm>
+#include <mutex>
 #include <new>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "tamx_internal.hpp"
@@ -35,19 +38,53 @@ namespace {
 constexpr int64_t kNormChunk = 1 << 16;  // elements per workgroup
 constexpr int kNormThreads = 256;
 
+// Where chunk c lies: a table built on the host (tamx_norm_plan) ...
+struct TableChunks {
+  const int64_t* __restrict__ cbeg;
+  const int64_t* __restrict__ cend;
+  __device__ bool operator()(int64_t c, int64_t& b, int64_t& e) const {
+    b = cbeg[c];
+    e = cend[c];
+    return true;
+  }
+};
+// ... or the per-tile chunk prefix tchunk (tchunk[ntiles] = the chunk count,
+// negative for an invalid table) and the tile table itself: chunk c is
+// chunk j = c - tchunk[t] of the tile t with tchunk[t] <= c < tchunk[t+1],
+// elements [j, j + 1) * kNormChunk of the tile clipped to its volume, the
+// same ranges the host table lists.
+struct PrefixChunks {
+  const int64_t* __restrict__ offsets;
+  const int64_t* __restrict__ volumes;
+  const int64_t* __restrict__ tchunk;
+  int64_t ntiles;
+  __device__ bool operator()(int64_t c, int64_t& b, int64_t& e) const {
+    if (!(c < tchunk[ntiles])) return false;
+    int64_t lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (tchunk[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const int64_t j = c - tchunk[lo], off = offsets[lo], vol = volumes[lo];
+    b = off + j * kNormChunk;
+    e = off + min(vol, (j + 1) * kNormChunk);
+    return true;
+  }
+};
+
 // Partial sums of squares, one workgroup per chunk.  Accumulation in the
 // tile's scalar type: double for FP64 tiles, float for FP32 tiles, as
 // Tensor<T>::squared_norm does (tensor/tensor.h:3768-3776).  With MAX, the
 // same pass also takes each chunk's max |a| (for the limb split's scale).
-template <typename T, bool MAX>
+template <typename T, bool MAX, class Chunks>
 __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
-    const T* __restrict__ pool, const int64_t* __restrict__ cbeg,
-    const int64_t* __restrict__ cend, double* __restrict__ partial,
+    const T* __restrict__ pool, Chunks chunks, double* __restrict__ partial,
     double* __restrict__ partial_max) {
   __shared__ T red[kNormThreads];
   __shared__ T redm[MAX ? kNormThreads : 1];
   const int64_t c = blockIdx.x;
-  const int64_t b = cbeg[c], e = cend[c];
+  int64_t b, e;
+  if (!chunks(c, b, e)) return;  // uniform over the workgroup
   // eight independent loads in flight per thread, eight partial sums
   constexpr int U = 8;
   T s[U];
@@ -90,9 +127,24 @@ __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
   }
 }
 
-// Combines the chunks of each tile in chunk order: norm = sqrt(sum), cast to
+// A tile's norm from its chunks [b, e) in chunk order: sqrt(sum), cast to
 // the float shape type (make_array stores into Tensor<float>,
-// conversions/make_array.h:170-172); with maxabs, also the tile's max |a|.
+// conversions/make_array.h:170-172); 0 for a tile with no chunk.
+template <typename T>
+__device__ inline float finish_norm(const double* __restrict__ partial, int64_t b, int64_t e) {
+  if (b == e) return 0.0f;
+  if (sizeof(T) == sizeof(double)) {
+    double s = 0.0;
+    for (int64_t c = b; c < e; ++c) s += partial[c];
+    return (float)sqrt(s);
+  }
+  float s = 0.0f;
+  for (int64_t c = b; c < e; ++c) s += (float)partial[c];
+  return sqrtf(s);
+}
+
+// Combines the chunks of each tile (finish_norm); with maxabs, also the
+// tile's max |a|.
 template <typename T>
 __global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
                                    const double* __restrict__ partial,
@@ -107,19 +159,7 @@ __global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
     for (int64_t c = b; c < e; ++c) m = fmax(m, partial_max[c]);
     maxabs[t] = m;
   }
-  if (b == e) {
-    norms[t] = 0.0f;
-    return;
-  }
-  if (sizeof(T) == sizeof(double)) {
-    double s = 0.0;
-    for (int64_t c = b; c < e; ++c) s += partial[c];
-    norms[t] = (float)sqrt(s);
-  } else {
-    float s = 0.0f;
-    for (int64_t c = b; c < e; ++c) s += (float)partial[c];
-    norms[t] = sqrtf(s);
-  }
+  norms[t] = finish_norm<T>(partial, b, e);
 }
 
 template <typename T>
@@ -133,10 +173,10 @@ int tile_norms(tamx_context* ctx, const tamx_norm_plan* plan, const T* pool,
   if (plan->nchunks > 0) {
     if (maxabs)
       norm_partial_kernel<T, true><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
-          pool, plan->cbeg, plan->cend, plan->partial, plan->partial_max);
+          pool, TableChunks{plan->cbeg, plan->cend}, plan->partial, plan->partial_max);
     else
       norm_partial_kernel<T, false><<<(unsigned)plan->nchunks, kNormThreads, 0, s>>>(
-          pool, plan->cbeg, plan->cend, plan->partial, nullptr);
+          pool, TableChunks{plan->cbeg, plan->cend}, plan->partial, nullptr);
     TAMX_LAUNCH_CHECK();
   }
   const unsigned grid = (unsigned)((plan->ntiles + 255) / 256);
@@ -145,6 +185,49 @@ int tile_norms(tamx_context* ctx, const tamx_norm_plan* plan, const T* pool,
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
 }
+
+// ------------------------------------------ norms without allocation --
+// tamx_tile_norms_f64_noalloc finds the chunks on the device (the same
+// ranges tamx_norm_plan_create lists on the host) and keeps the chunk prefix
+// and the partial sums in the file-scope `scratch` array below.
+
+// tchunk[t] = exclusive prefix over tiles of ceil(volume / kNormChunk)
+// (absent or empty tiles have none); tchunk[ntiles] = the total, or -1 when
+// the total exceeds max_chunks or a present tile has a negative volume.
+__global__ __launch_bounds__(1024) void norm_chunk_scan_kernel(
+    const int64_t* __restrict__ offsets, const int64_t* __restrict__ volumes,
+    int64_t ntiles, int64_t max_chunks, int64_t* __restrict__ tchunk) {
+  __shared__ int64_t part[1024];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const int64_t total = tamx::block_exclusive_scan<1024, int64_t>(
+      ntiles, part,
+      [&](int64_t t) -> int64_t {
+        const int64_t o = offsets[t], v = volumes[t];
+        if (o < 0) return 0;
+        if (v < 0) {
+          bad = 1;
+          return 0;
+        }
+        return (v + kNormChunk - 1) / kNormChunk;
+      },
+      [&](int64_t t, const int64_t& x) { tchunk[t] = x; });
+  if (threadIdx.x == 0) tchunk[ntiles] = (bad || total > max_chunks) ? -1 : total;
+}
+
+// finish_norm per tile, NaN for every tile of an invalid table.
+template <typename T>
+__global__ void norm_finish_prefix_kernel(const int64_t* __restrict__ tchunk,
+                                          const double* __restrict__ partial, int64_t ntiles,
+                                          float* __restrict__ norms) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  norms[t] = tchunk[ntiles] < 0 ? __builtin_nanf("") : finish_norm<T>(partial, tchunk[t], tchunk[t + 1]);
+}
+
+constexpr int64_t kNoallocPrefixBytes = ((TAMX_NOALLOC_MAX_TILES + 1) * 8 + 255) / 256 * 256;
+constexpr int64_t kNoallocScratchBytes = kNoallocPrefixBytes + TAMX_NOALLOC_MAX_CHUNKS * 8;
 
 // ------------------------------------------------------- shape scaling --
 
@@ -481,6 +564,31 @@ __global__ __launch_bounds__(kWlThreads) void wl_fill_kernel(WlParams p) {
 
 }  // namespace
 
+// Scratch of tamx_tile_norms_f64_noalloc: the chunk prefix, then one partial
+// sum per chunk (about 10 MiB).  Every GPU the library runs on holds its own
+// copy until the process ends.  In host code this name is a stand-in, not a
+// device address: the device address comes from hipGetSymbolAddress
+// (norm_scratch_address), once per device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace tamx {
+// The device address of `scratch` on ctx's device (current when called):
+// resolved once per device, at context creation.
+int norm_scratch_address(tamx_context* ctx) {
+  static std::mutex mu;
+  static std::unordered_map<int, void*> addr;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = addr.find(ctx->device);
+  if (it == addr.end()) {
+    void* ptr = nullptr;
+    TAMX_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    it = addr.emplace(ctx->device, ptr).first;
+  }
+  ctx->norm_scratch = it->second;
+  return TAMX_OK;
+}
+}  // namespace tamx
+
 extern "C" {
 
 int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
@@ -563,6 +671,47 @@ int tamx_tile_norms_max_f64(tamx_context* ctx, const tamx_norm_plan* plan,
 int tamx_tile_norms_f32(tamx_context* ctx, const tamx_norm_plan* plan,
                         const float* pool, float* norms, void* stream) {
   return tile_norms<float>(ctx, plan, pool, norms, nullptr, stream);
+}
+
+int tamx_tile_norms_f64_noalloc(tamx_context* ctx, const int64_t* offsets,
+                                const int64_t* volumes, int64_t ntiles, const double* pool,
+                                int64_t pool_elems, float* norms, void* stream) {
+  TAMX_REQUIRE(ctx != nullptr, "null context");
+  TAMX_REQUIRE(ntiles >= 0 && pool_elems >= 0, "negative size");
+  if (ntiles == 0) return TAMX_OK;
+  TAMX_REQUIRE(offsets != nullptr && volumes != nullptr && norms != nullptr, "null argument");
+  TAMX_REQUIRE(pool != nullptr || pool_elems == 0, "null pool");
+  if (ntiles > TAMX_NOALLOC_MAX_TILES) {
+    tamx::set_error("tamx_tile_norms_f64_noalloc: " + std::to_string(ntiles) +
+                    " tiles exceed the limit of " + std::to_string(TAMX_NOALLOC_MAX_TILES));
+    return TAMX_ERR_CAPACITY;
+  }
+  // tiles lying inside the pool have at most this many chunks in all
+  const int64_t max_chunks = pool_elems / kNormChunk + ntiles;
+  if (max_chunks > TAMX_NOALLOC_MAX_CHUNKS) {
+    tamx::set_error("tamx_tile_norms_f64_noalloc: a pool of " + std::to_string(pool_elems) +
+                    " elements in " + std::to_string(ntiles) + " tiles may need " +
+                    std::to_string(max_chunks) + " chunks of 65536, beyond the limit of " +
+                    std::to_string(TAMX_NOALLOC_MAX_CHUNKS));
+    return TAMX_ERR_CAPACITY;
+  }
+  TAMX_REQUIRE(ctx->norm_scratch != nullptr, "context without a scratch address");
+  if (int rc = tamx::activate(ctx)) return rc;
+  hipStream_t s = tamx::as_stream(stream);
+  auto* tchunk = static_cast<int64_t*>(ctx->norm_scratch);
+  auto* partial = reinterpret_cast<double*>(static_cast<char*>(ctx->norm_scratch) +
+                                            kNoallocPrefixBytes);
+  norm_chunk_scan_kernel<<<1, 1024, 0, s>>>(offsets, volumes, ntiles, max_chunks, tchunk);
+  TAMX_LAUNCH_CHECK();
+  if (max_chunks > 0) {
+    norm_partial_kernel<double, false><<<(unsigned)max_chunks, kNormThreads, 0, s>>>(
+        pool, PrefixChunks{offsets, volumes, tchunk, ntiles}, partial, nullptr);
+    TAMX_LAUNCH_CHECK();
+  }
+  norm_finish_prefix_kernel<double><<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(
+      tchunk, partial, ntiles, norms);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
 }
 
 int tamx_shape_scale(tamx_context* ctx, float* norms, int rank,
