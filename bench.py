@@ -371,7 +371,7 @@ def main():
             if st is None and args.tile == "auto" and prec.startswith(("bf16", "i8")):
                 make_step(cs)(None)
                 torch.cuda.synchronize()
-                st = tuple(cs.tune(reps=2)[0][:2])
+                st = tuple(cs.tune(reps=2)[0][:3])
             ms_s, gemm_s = time_steps(cs, make_step(cs), args.steps, args.warmup)
             g = cs.C.cpu().numpy()[used]
             split[prec] = {

@@ -4,7 +4,10 @@
  * Plain C types only: device pointers, sizes and an opaque context.  A stream
  * argument is a hipStream_t passed as void* (NULL = the default stream).
  * Every entry point returns a tamx_status; tamx_last_error() returns a
- * thread-local message describing the last failure.
+ * thread-local message describing the last failure.  Entry points that
+ * touch the device first clear HIP's pending last-error state, so a failure
+ * already reported (for example an allocation returned as TAMX_ERR_OOM) is
+ * not taken for a later launch's.
  *
  * Conventions shared with TiledArray:
  *   - tiles are dense row-major blocks (TA::Tensor storage, tensor/tensor.h:454-458);
@@ -371,6 +374,33 @@ int tamx_grouped_gemm_gather_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
                                  const tamx_gather_plan* b_gather, float* C,
                                  const tamx_gemm_config* cfg, void* stream);
 
+/* Result tiles stored through an index map.  Replaces the permute of each
+ * finished result tile into the target order (ContractReduce's post-op,
+ * tile_op/contract_reduce.h:370-378, with the shape's permute,
+ * sparse_shape.h:1687-1691): the GEMM writes every element of the natural
+ * (left outer, right outer) result tile straight to its place in the
+ * permuted tile.  c_scatter is a gather plan over the result tile grid, by
+ * the grid position tamx_contraction_worklist gives each task (batch-major,
+ * then row-major over Mt x Nt): its row modes are the left-outer modes,
+ * its k modes the right-outer modes, each with its stride in the stored
+ * target-order tile, so element (row, col) of result tile g lies at the
+ * task's c_off + rowoff(row) + koff(col).  The work list must come from
+ * tamx_contraction_worklist, which records each task's grid position; the
+ * task's ldc and STORE_TRANS flag are unused.  Operand gathers as in
+ * tamx_grouped_gemm_gather_*. */
+int tamx_grouped_gemm_scatter_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                  int op_b, double alpha, const double* A,
+                                  const tamx_gather_plan* a_gather, const double* B,
+                                  const tamx_gather_plan* b_gather, double* C,
+                                  const tamx_gather_plan* c_scatter,
+                                  const tamx_gemm_config* cfg, void* stream);
+int tamx_grouped_gemm_scatter_f32(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                  int op_b, float alpha, const float* A,
+                                  const tamx_gather_plan* a_gather, const float* B,
+                                  const tamx_gather_plan* b_gather, float* C,
+                                  const tamx_gather_plan* c_scatter,
+                                  const tamx_gemm_config* cfg, void* stream);
+
 /* ------------------------------------------------ split precision --- */
 
 /* Splitting of FP64 operand tiles into nslices BF16 limbs with one
@@ -465,6 +495,13 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                                    const tamx_split_plan* a_plan, const void* a_limbs,
                                    const tamx_split_plan* b_plan, const void* b_limbs,
                                    double* C, const tamx_gemm_config* cfg, void* stream);
+/* The same storing result tiles through a scatter plan (as
+ * tamx_grouped_gemm_scatter_f64; null: as _config). */
+int tamx_grouped_gemm_split_scatter(tamx_context* ctx, tamx_worklist* wl, double alpha,
+                                    const tamx_split_plan* a_plan, const void* a_limbs,
+                                    const tamx_split_plan* b_plan, const void* b_limbs,
+                                    double* C, const tamx_gather_plan* c_scatter,
+                                    const tamx_gemm_config* cfg, void* stream);
 
 /* Two FP32 limbs: FP64 tiles split while staged (hi = fp32(a),
  * lo = fp32(a - hi)); hi*hi and hi*lo + lo*hi on FP32 MFMA with FP32
@@ -473,6 +510,12 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
 int tamx_grouped_gemm_f32x2(tamx_context* ctx, tamx_worklist* wl, int op_a,
                             int op_b, double alpha, const double* A,
                             const double* B, double* C, void* stream);
+/* The same storing result tiles through a scatter plan (as
+ * tamx_grouped_gemm_scatter_f64; null: as tamx_grouped_gemm_f32x2). */
+int tamx_grouped_gemm_f32x2_scatter(tamx_context* ctx, tamx_worklist* wl, int op_a,
+                                    int op_b, double alpha, const double* A,
+                                    const double* B, double* C,
+                                    const tamx_gather_plan* c_scatter, void* stream);
 
 /* ------------------------------------------------------ autotuning --- */
 

@@ -339,3 +339,86 @@ def test_incongruent_tilings_raise(ctx):
     B = DistArray.allocate(TiledRange([[0, 4, 8], [0, 8]]))
     with pytest.raises(ValueError):
         E.Contraction(ctx, "m,n", A, "m,k", B, "k,n")
+
+
+@pytest.mark.parametrize("short", ["tasks", "pairs", "c_elements"])
+def test_worklist_capacity_overflow(ctx, short):
+    # A work list (or result pool) too small for the schedule: the builder
+    # marks TAMX_ERR_CAPACITY on the device, hands the GEMM an empty list and
+    # every result tile offset -1, so C is left as it was and every kernel
+    # exits normally (TA throws on the same resource limits,
+    # contraction_eval.h:1855-1898); the host reads the mark as an error.
+    from tiledarray_amd import _lib
+    rng = np.random.default_rng(21)
+    trm = TiledRange1([0, 40, 130, 200])
+    trk = TiledRange1([0, 17, 64, 150])
+    trn = TiledRange1([0, 96, 97, 250])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, 0, 0, torch.float64, True)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n")
+    grid = c.Mt * c.Nt
+    if short == "tasks":
+        c.wl = E.WorkList(ctx, grid - 1, grid * c.Kt, grid)
+    elif short == "pairs":
+        c.wl = E.WorkList(ctx, grid, grid * c.Kt - 1, grid)
+    else:
+        c._desc.c_capacity = c.c_capacity - 1
+    c.C.fill_(7.0)
+    c.run()
+    torch.cuda.synchronize()
+    cnt = c.wl.counters()
+    assert cnt[_lib.WL_STATUS] == _lib.TAMX_ERR_CAPACITY
+    assert cnt[_lib.WL_NTASKS] == 0 and cnt[_lib.WL_NPAIRS] == 0
+    # the totals that were asked for
+    assert cnt[5] == grid and cnt[6] == grid * c.Kt
+    assert bool((c.C == 7.0).all())
+    assert bool((c.c_off == -1).all())
+    with pytest.raises(_lib.TamxError) as ei:
+        c.counters()
+    assert ei.value.status == _lib.TAMX_ERR_CAPACITY
+    # the same context and kernels then run a list that fits
+    c.wl = E.WorkList(ctx, grid, grid * c.Kt, grid)
+    c._desc.c_capacity = c.c_capacity
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+    assert c.counters()["npairs"] == npairs
+
+
+def test_grid_capacity_and_out_of_memory_status(ctx):
+    # a result grid beyond the work list's grid capacity is refused on the
+    # host before any launch; a work list beyond device memory returns
+    # TAMX_ERR_OOM with a message, and allocations work again after it
+    import ctypes
+    from tiledarray_amd import _lib
+    lib = E.lib()
+    small = E.WorkList(ctx, 1, 1, 1)
+    rng = np.random.default_rng(3)
+    trm = TiledRange1([0, 8, 16])
+    A, B = matrix_ops_arrays(rng, trm, trm, trm, 0, 0, torch.float64, True)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n")
+    c.shape_step()
+    rc = lib.tamx_contraction_worklist(ctx.ptr, ctypes.byref(c._desc), small.ptr, ctx.stream())
+    assert rc == _lib.TAMX_ERR_CAPACITY
+    assert b"grid capacity" in lib.tamx_last_error()
+    # about 215 GB per list at the largest capacities: the second or third
+    # cannot fit in one GPU's memory
+    cap = (1 << 31) - 2
+    made, status = [], _lib.TAMX_OK
+    for _ in range(3):
+        p = ctypes.c_void_p()
+        status = lib.tamx_worklist_create(ctx.ptr, cap, cap, cap, ctypes.byref(p))
+        if status != _lib.TAMX_OK:
+            break
+        made.append(p)
+    msg = lib.tamx_last_error()
+    for p in made:
+        assert lib.tamx_worklist_destroy(p) == _lib.TAMX_OK
+    assert status == _lib.TAMX_ERR_OOM, (status, msg)
+    assert b"allocation" in msg
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])

@@ -125,6 +125,50 @@ def test_split_wide_tiles_match_default(ctx, tile, precision):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
+@pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b):
+    # the two-wave INT8 kernel (64 x 64 macro tiles, bk 128 selects it) on
+    # tiles ragged in both directions, narrower and wider than a macro tile
+    nslices = int(precision.split("x")[-1])
+    rng = np.random.default_rng(1300 + nslices * 4 + op_a * 2 + op_b)
+    trm = TiledRange1([0, 40, 170, 300, 301])
+    trk = TiledRange1([0, 17, 64, 150, 151, 300])
+    trn = TiledRange1([0, 96, 97, 250])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
+    la = "k,m" if op_a else "m,k"
+    lb = "n,k" if op_b else "k,n"
+    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=precision,
+                      tile=(64, 64, 128))
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+@pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
+def test_split_i8_two_wave_matches_default(ctx, precision):
+    # same limbs, same pair order, same folds: the two-wave kernel's result
+    # equals the four-wave kernel's bit for bit on random block-sparse data,
+    # and on a second run over a smaller problem after it
+    for n, seed in ((1500, 31), (700, 32)):
+        A, B, _ = W.blocksparse_gemm(n=n, wlo=24, whi=150, seed=seed)
+        E.make_sparse(ctx, A, 1e-8)
+        E.make_sparse(ctx, B, 1e-8)
+        c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8,
+                          precision=precision, alpha=-1.5)
+        c.run()
+        torch.cuda.synchronize()
+        C0 = c.C.clone()
+        c.set_tile((64, 64, 128))
+        c.C.zero_()
+        c.gemm_step()
+        torch.cuda.synchronize()
+        mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
+        assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
+
+
 def test_split_sigma_is_tile_exponent(ctx):
     rng = np.random.default_rng(5)
     tr = TiledRange1([0, 20, 45])

@@ -152,6 +152,12 @@ SIGNATURES = {
     "tamx_gather_plan_destroy": [_vp],
     "tamx_grouped_gemm_gather_f64": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tamx_grouped_gemm_gather_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_scatter_f64": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp],
+    "tamx_grouped_gemm_scatter_f32": [_vp, _vp, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp],
+    "tamx_grouped_gemm_split_scatter": [_vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tamx_grouped_gemm_f32x2_scatter": [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _vp],
     "tamx_autotune_grouped_gemm_gather": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i,
                                           _vp, _vp, _vp],
     "tamx_graph_begin": [_vp, _vp],

@@ -35,7 +35,8 @@ FILE_FLAGS = {"tamx_shape.hip": ["-ffp-contract=off"],
               "tamx_gemm_gather.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
               "tamx_gemm_gather_f32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
               "tamx_split.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
-              "tamx_split_wide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+              "tamx_split_wide.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "tamx_split_i8.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src, obj, verbose):

@@ -13,6 +13,11 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 int activate(const Context* ctx) {
   TAMX_REQUIRE(ctx != nullptr, "null context");
+  // HIP keeps the last failed call's error until hipGetLastError reads it;
+  // a failure the caller (or this library, e.g. an allocation reported as
+  // TAMX_ERR_OOM) has already seen must not be taken for this entry
+  // point's launch failure
+  (void)hipGetLastError();
   int cur = -1;
   TAMX_CHECK_HIP(hipGetDevice(&cur));
   if (cur != ctx->device) TAMX_CHECK_HIP(hipSetDevice(ctx->device));
@@ -112,6 +117,7 @@ int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
   alloc(&wl->tile_volume, sizeof(int64_t) * gc);
   alloc(&wl->tile_flops, sizeof(int64_t) * gc);
   alloc(&wl->item_prefix, sizeof(int32_t) * (tc + 1));
+  alloc(&wl->task_tile, sizeof(int32_t) * tc);
   // room for 16 macro tiles per task; larger lists fall back to a binary
   // search over item_prefix inside the GEMM kernel
   wl->item_capacity = std::min<int64_t>(tc * 16, (int64_t(1) << 31) - 1);
@@ -135,7 +141,8 @@ int tamx_worklist_destroy(tamx_worklist* wl) {
   if (!wl) return TAMX_OK;
   void* ptrs[] = {wl->tasks,       wl->pairs,          wl->counters,
                   wl->tile_npairs, wl->tile_volume,    wl->tile_flops,
-                  wl->item_prefix, wl->item_task, wl->queue, wl->scan_partials};
+                  wl->item_prefix, wl->item_task, wl->queue, wl->scan_partials,
+                  wl->task_tile};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete wl;

@@ -88,6 +88,26 @@ int tamx_gemm_config_get(int index, tamx_gemm_config* cfg) {
   return TAMX_OK;
 }
 
+}  // extern "C"
+
+int tamx::plain_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
+                         double alpha, const double* A, const double* B, double* C,
+                         const tamx_gemm_config* cfg, const GatherTile* c_scatter,
+                         void* stream) {
+  return grouped_gemm<double, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, nullptr,
+                                 nullptr, c_scatter);
+}
+
+int tamx::plain_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
+                         float alpha, const float* A, const float* B, float* C,
+                         const tamx_gemm_config* cfg, const GatherTile* c_scatter,
+                         void* stream) {
+  return grouped_gemm<float, 0>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, nullptr,
+                                nullptr, c_scatter);
+}
+
+extern "C" {
+
 int tamx_grouped_gemm_f64_config(tamx_context* ctx, tamx_worklist* wl, int op_a,
                                  int op_b, double alpha, const double* A,
                                  const double* B, double* C,

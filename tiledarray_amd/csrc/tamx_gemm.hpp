@@ -23,54 +23,12 @@
 #include <unordered_map>
 
 #include "tamx_internal.hpp"
-
-namespace tamx {
-// One mode of a gathered tile's index map: extent, stride in the stored
-// tile, and the constants that divide by the extent with a multiply and
-// shifts (Granlund and Montgomery, "Division by invariant integers using
-// multiplication", 1994, Fig. 4.1): with l = ceil(log2 ext),
-// magic = floor(2^32 (2^l - ext) / ext) + 1, t = mulhi(magic, x),
-// x / ext = (t + ((x - t) >> min(l, 1))) >> max(l - 1, 0), exact for every
-// 32-bit x.
-struct GatherMode {
-  uint32_t magic;
-  uint32_t shifts;  // min(l, 1) | max(l - 1, 0) << 8
-  int32_t ext;
-  int32_t str;
-};
-__host__ __device__ inline uint32_t gather_div(uint32_t u, const GatherMode& m) {
-  const uint32_t t = (uint32_t)(((uint64_t)m.magic * u) >> 32);
-  return (t + ((u - t) >> (m.shifts & 0xff))) >> (m.shifts >> 8);
-}
-// The mode (ext, str) with its division constants (ext 0 divides as 1).
-inline GatherMode make_gather_mode(int32_t ext, int32_t str) {
-  const uint64_t d = ext > 0 ? (uint64_t)ext : 1u;
-  int l = 0;
-  while ((uint64_t(1) << l) < d) ++l;
-  GatherMode m;
-  m.magic = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
-  m.shifts = (uint32_t)(l < 1 ? l : 1) | ((uint32_t)(l > 1 ? l - 1 : 0) << 8);
-  m.ext = ext;
-  m.str = str;
-  return m;
-}
-// Index map of one gathered tile: element (row, k) lies at the tile's offset
-// + rowoff(row) + koff(k), each a mixed-radix sum over four modes (the last
-// fastest) with those modes' strides in the stored tile; a map of fewer
-// source modes is padded at the slow end with (extent 1, stride 0) modes.
-// Offsets inside a tile stay below 2^31.
-struct GatherTile {
-  GatherMode r[4], k[4];
-};
-}  // namespace tamx
-
-struct tamx_gather_plan {
-  tamx_context* ctx = nullptr;
-  int64_t ntiles = 0;
-  tamx::GatherTile* tiles = nullptr;
-};
+#include "tamx_map.hpp"
 
 namespace {
+
+using tamx::gather_offset;
+using tamx::load_modes;
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -111,6 +69,10 @@ struct GemmParams {
   // gathered operands (null: plain row-major tiles with the pair's ld)
   const tamx::GatherTile* __restrict__ a_gather;
   const tamx::GatherTile* __restrict__ b_gather;
+  // result tiles stored through a scatter map (null: row-major with the
+  // task's ldc); task_tile[t] is task t's position in the result grid
+  const tamx::GatherTile* __restrict__ c_scatter;
+  const int32_t* __restrict__ task_tile;
 };
 
 constexpr int kThreads = 256;
@@ -132,31 +94,6 @@ __device__ inline int64_t uni64(int64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
   return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// Offset of element x of a gathered tile's row (or k) index: the
-// mixed-radix sum over the four modes in m (wave-uniform), the last
-// fastest; three multiply-shift divisions, no branches.
-__device__ inline int32_t gather_offset(int x, const tamx::GatherMode (&m)[4]) {
-  uint32_t off = 0, u = (uint32_t)x;
-#pragma unroll
-  for (int d = 3; d >= 1; --d) {
-    const uint32_t q = tamx::gather_div(u, m[d]);
-    off += (u - q * (uint32_t)m[d].ext) * (uint32_t)m[d].str;
-    u = q;
-  }
-  return (int32_t)(off + u * (uint32_t)m[0].str);
-}
-
-__device__ inline void load_modes(const __attribute__((address_space(4))) tamx::GatherMode* src,
-                                  tamx::GatherMode (&dst)[4]) {
-#pragma unroll
-  for (int d = 0; d < 4; ++d) {
-    dst[d].magic = src[d].magic;
-    dst[d].shifts = src[d].shifts;
-    dst[d].ext = src[d].ext;
-    dst[d].str = src[d].str;
-  }
 }
 
 // GM: which operands the launch gathers (bit 0 A, bit 1 B; 0 for the plain
@@ -468,24 +405,24 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     }
 
     const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
     T* __restrict__ c = p.C + t_coff;
+    tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = wr + i * 16 + M::row(lk, v);
-          const int col = wc + j * 16 + lrow;
-          if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
-                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
-            T val = p.alpha * acc[i][j][v];
-            if (beta1) val += c[idx];
-            c[idx] = val;
+          for (int v = 0; v < 4; ++v) {
+            const int row = wr + i * 16 + M::row(lk, v);
+            const int col = wc + j * 16 + lrow;
+            if (row < mrem && col < nrem) {
+              const int64_t idx = cidx(r0 + row, c0 + col);
+              T val = p.alpha * acc[i][j][v];
+              if (beta1) val += c[idx];
+              c[idx] = val;
+            }
           }
-        }
+    });
   }
 
   if (tid == 0) {
@@ -550,14 +487,16 @@ template <typename T, int GM>
 int grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
                  T alpha, const T* A, const T* B, T* C, void* stream,
                  const tamx_gemm_config* cfg, const tamx::GatherTile* a_gather = nullptr,
-                 const tamx::GatherTile* b_gather = nullptr) {
+                 const tamx::GatherTile* b_gather = nullptr,
+                 const tamx::GatherTile* c_scatter = nullptr) {
   TAMX_REQUIRE(wl != nullptr, "null work list");
   TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be TAMX_NO_TRANS or TAMX_TRANS");
   TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be TAMX_NO_TRANS or TAMX_TRANS");
   if (int rc = tamx::activate(ctx)) return rc;
   GemmParams<T> prm{A, B, C, wl->tasks, wl->pairs, wl->counters,
                     wl->item_prefix, wl->item_task, wl->queue,
-                    wl->task_capacity, wl->item_capacity, alpha, a_gather, b_gather};
+                    wl->task_capacity, wl->item_capacity, alpha, a_gather, b_gather,
+                    c_scatter, wl->task_tile};
   hipStream_t s = tamx::as_stream(stream);
   const int bm = cfg ? cfg->bm : 64, bn = cfg ? cfg->bn : 64, bk = cfg ? cfg->bk : 16;
   const int w = cfg ? cfg->wgs_per_cu : 0;
@@ -581,32 +520,52 @@ int grouped_gemm(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
   return TAMX_ERR_UNSUPPORTED;
 }
 
+}  // namespace
+
+namespace tamx {
+// The plain-operand launches (tamx_gemm.hip), with an optional result
+// scatter plan.
+int plain_gemm_f64(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b, double alpha,
+                   const double* A, const double* B, double* C, const tamx_gemm_config* cfg,
+                   const GatherTile* c_scatter, void* stream);
+int plain_gemm_f32(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b, float alpha,
+                   const float* A, const float* B, float* C, const tamx_gemm_config* cfg,
+                   const GatherTile* c_scatter, void* stream);
+}  // namespace tamx
+
+namespace {
+
 inline int plain_config(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b, double alpha,
                         const double* A, const double* B, double* C,
-                        const tamx_gemm_config* cfg, void* stream) {
-  return tamx_grouped_gemm_f64_config(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, stream);
+                        const tamx_gemm_config* cfg, const tamx::GatherTile* c_scatter,
+                        void* stream) {
+  return tamx::plain_gemm_f64(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, c_scatter, stream);
 }
 inline int plain_config(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b, float alpha,
                         const float* A, const float* B, float* C,
-                        const tamx_gemm_config* cfg, void* stream) {
-  return tamx_grouped_gemm_f32_config(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, stream);
+                        const tamx_gemm_config* cfg, const tamx::GatherTile* c_scatter,
+                        void* stream) {
+  return tamx::plain_gemm_f32(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, c_scatter, stream);
 }
 
-// One launch with gathered operands (null plans: plain operands).
+// One launch with gathered operands (null plans: plain operands) and an
+// optional result scatter plan.
 template <typename T>
 int gemm_gathered(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b, T alpha,
                   const T* A, const tamx_gather_plan* a_gather, const T* B,
-                  const tamx_gather_plan* b_gather, T* C, const tamx_gemm_config* cfg,
-                  void* stream) {
+                  const tamx_gather_plan* b_gather, T* C, const tamx_gather_plan* c_scatter,
+                  const tamx_gemm_config* cfg, void* stream) {
   TAMX_REQUIRE(a_gather == nullptr || a_gather->ctx == ctx, "A gather plan of another context");
   TAMX_REQUIRE(b_gather == nullptr || b_gather->ctx == ctx, "B gather plan of another context");
+  TAMX_REQUIRE(c_scatter == nullptr || c_scatter->ctx == ctx, "scatter plan of another context");
   const tamx::GatherTile* ga = a_gather ? a_gather->tiles : nullptr;
   const tamx::GatherTile* gb = b_gather ? b_gather->tiles : nullptr;
+  const tamx::GatherTile* gc = c_scatter ? c_scatter->tiles : nullptr;
   if (ga && gb)
-    return grouped_gemm<T, 3>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb);
-  if (ga) return grouped_gemm<T, 1>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb);
-  if (gb) return grouped_gemm<T, 2>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb);
-  return plain_config(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, stream);
+    return grouped_gemm<T, 3>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb, gc);
+  if (ga) return grouped_gemm<T, 1>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb, gc);
+  if (gb) return grouped_gemm<T, 2>(ctx, wl, op_a, op_b, alpha, A, B, C, stream, cfg, ga, gb, gc);
+  return plain_config(ctx, wl, op_a, op_b, alpha, A, B, C, cfg, gc, stream);
 }
 
 }  // namespace

@@ -75,7 +75,18 @@ int tamx_grouped_gemm_gather_f64(tamx_context* ctx, tamx_worklist* wl, int op_a,
                                  const tamx_gather_plan* b_gather, double* C,
                                  const tamx_gemm_config* cfg, void* stream) {
   return gemm_gathered<double>(ctx, wl, op_a, op_b, alpha, A, a_gather, B, b_gather, C,
-                                     cfg, stream);
+                               nullptr, cfg, stream);
+}
+
+int tamx_grouped_gemm_scatter_f64(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
+                                  double alpha, const double* A,
+                                  const tamx_gather_plan* a_gather, const double* B,
+                                  const tamx_gather_plan* b_gather, double* C,
+                                  const tamx_gather_plan* c_scatter,
+                                  const tamx_gemm_config* cfg, void* stream) {
+  TAMX_REQUIRE(c_scatter != nullptr, "null scatter plan");
+  return gemm_gathered<double>(ctx, wl, op_a, op_b, alpha, A, a_gather, B, b_gather, C,
+                               c_scatter, cfg, stream);
 }
 
 }  // extern "C"

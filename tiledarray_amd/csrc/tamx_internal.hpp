@@ -63,6 +63,9 @@ struct tamx_worklist {
   // scratch of the grouped GEMM: macro-tile prefix over tasks and the
   // persistent-kernel queue (next item, finished workgroups)
   int32_t* item_prefix = nullptr;  // task_capacity + 1
+  // position of each task's result tile in the grid (written by
+  // tamx_contraction_worklist; read by launches with a scatter plan)
+  int32_t* task_tile = nullptr;
   int32_t* item_task = nullptr;    // item -> task map, item_capacity entries
   int64_t item_capacity = 0;
   int32_t* queue = nullptr;        // 16 int32 (per-XCD next item, done count), zero between launches

@@ -101,14 +101,54 @@ __device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ d
     T* pl = plane;
     const int ld = ns | 1;
     const int n = ns * nt;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-      const int y = e / ns, x = e - y * ns;  // t index, s index
-      pl[y * ld + x] = src[so + (int64_t)y * t.sstride[td] + x];
+    // element e = tid + i * blockDim of the plane is (e / w, e % w) for a row
+    // length w; one division per item, then each step of blockDim elements
+    // advances by (blockDim / w, blockDim % w) with one carry, and four
+    // elements per thread are loaded before any is stored
+    constexpr int U = 4;
+    {
+      const int qs = blockDim.x / ns, rs = blockDim.x - qs * ns;
+      int y = threadIdx.x / ns, x = threadIdx.x - y * ns;  // t index, s index
+      const T* __restrict__ sp = src + so;
+      const int64_t st = t.sstride[td];
+      for (int e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
+        T v[U];
+        int yy[U], xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          yy[u] = y;
+          xx[u] = x;
+          v[u] = e0 + u * (int)blockDim.x < n ? sp[(int64_t)y * st + x] : T(0);
+          x += rs;
+          y += qs;
+          if (x >= ns) {
+            x -= ns;
+            ++y;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (e0 + u * (int)blockDim.x < n) pl[yy[u] * ld + xx[u]] = v[u];
+      }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-      const int y = e / nt, x = e - y * nt;  // s index, t index
-      dst[doff + (int64_t)y * t.dstride[sd] + x] = pl[x * ld + y];
+    {
+      const int qt = blockDim.x / nt, rt = blockDim.x - qt * nt;
+      int y = threadIdx.x / nt, x = threadIdx.x - y * nt;  // s index, t index
+      T* __restrict__ dp = dst + doff;
+      const int64_t st = t.dstride[sd];
+      for (int e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (e0 + u * (int)blockDim.x < n) dp[(int64_t)y * st + x] = pl[x * ld + y];
+          x += rt;
+          y += qt;
+          if (x >= nt) {
+            x -= nt;
+            ++y;
+          }
+        }
+      }
     }
     return;
   }

@@ -385,6 +385,7 @@ constexpr int kWlThreads = 256;  // result tiles per workgroup of count / fill
 struct WlParams {
   tamx_contraction_desc d;
   tamx_gemm_task* tasks;
+  int32_t* task_tile;
   tamx_gemm_pair* pairs;
   int64_t* counters;
   int32_t* tile_npairs;
@@ -535,6 +536,7 @@ __global__ __launch_bounds__(kWlThreads) void wl_fill_kernel(WlParams p) {
   task.pair_count = p.tile_npairs[g];
   task.flags = d.store_trans ? TAMX_TASK_STORE_TRANS : 0;
   p.tasks[t] = task;
+  p.task_tile[t] = (int32_t)g;
   const float thr_k =
       d.pair_threshold > 0.0f ? d.pair_threshold / (float)d.Kt : 0.0f;
   int32_t q = pb;
@@ -786,6 +788,7 @@ int tamx_contraction_worklist(tamx_context* ctx,
   if (int rc = tamx::activate(ctx)) return rc;
   WlParams p{d,
              wl->tasks,
+             wl->task_tile,
              wl->pairs,
              wl->counters,
              wl->tile_npairs,

@@ -666,25 +666,25 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     }
 
     const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
     double* __restrict__ c = p.C + t_coff;
+    tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          // f32-output 16x16 MFMA layout: row 4 * (lane / 16) + v, column lane % 16
-          const int row = wr + i * 16 + lq * 4 + v;
-          const int col = wc + j * 16 + lrow;
-          if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
-                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
-            double val = p.alpha * acc[i][j][v];
-            if (beta1) val += c[idx];
-            c[idx] = val;
+          for (int v = 0; v < 4; ++v) {
+            // f32-output 16x16 MFMA layout: row 4 * (lane / 16) + v, column lane % 16
+            const int row = wr + i * 16 + lq * 4 + v;
+            const int col = wc + j * 16 + lrow;
+            if (row < mrem && col < nrem) {
+              const int64_t idx = cidx(r0 + row, c0 + col);
+              double val = p.alpha * acc[i][j][v];
+              if (beta1) val += c[idx];
+              c[idx] = val;
+            }
           }
-        }
+    });
   }
 
   if (tid == 0) {
@@ -932,24 +932,24 @@ __global__ __launch_bounds__(kThreads) void f32x2_gemm_kernel(
       }
     }
     const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
     double* __restrict__ c = p.C + t_coff;
+    tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = wr + i * 16 + lk * 4 + v;  // f32 16x16 MFMA layout
-          const int col = wc + j * 16 + lrow;
-          if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
-                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
-            double val = p.alpha * acc[i][j][v];
-            if (beta1) val += c[idx];
-            c[idx] = val;
+          for (int v = 0; v < 4; ++v) {
+            const int row = wr + i * 16 + lk * 4 + v;  // f32 16x16 MFMA layout
+            const int col = wc + j * 16 + lrow;
+            if (row < mrem && col < nrem) {
+              const int64_t idx = cidx(r0 + row, c0 + col);
+              double val = p.alpha * acc[i][j][v];
+              if (beta1) val += c[idx];
+              c[idx] = val;
+            }
           }
-        }
+    });
   }
   if (tid == 0) {
     __threadfence();
@@ -1246,7 +1246,17 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                                    const tamx_split_plan* a_plan, const void* a_limbs,
                                    const tamx_split_plan* b_plan, const void* b_limbs,
                                    double* C, const tamx_gemm_config* cfg, void* stream) {
+  return tamx_grouped_gemm_split_scatter(ctx, wl, alpha, a_plan, a_limbs, b_plan, b_limbs, C,
+                                         nullptr, cfg, stream);
+}
+
+int tamx_grouped_gemm_split_scatter(tamx_context* ctx, tamx_worklist* wl, double alpha,
+                                    const tamx_split_plan* a_plan, const void* a_limbs,
+                                    const tamx_split_plan* b_plan, const void* b_limbs,
+                                    double* C, const tamx_gather_plan* c_scatter,
+                                    const tamx_gemm_config* cfg, void* stream) {
   TAMX_REQUIRE(wl && a_plan && b_plan && a_limbs && b_limbs, "null argument");
+  TAMX_REQUIRE(c_scatter == nullptr || c_scatter->ctx == ctx, "scatter plan of another context");
   TAMX_REQUIRE(a_plan->nslices == b_plan->nslices, "operands split into different limb counts");
   TAMX_REQUIRE(a_plan->limb_type == b_plan->limb_type, "operands split into different limb types");
   if (int rc = tamx::activate(ctx)) return rc;
@@ -1263,9 +1273,13 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
                   wl->queue,
                   wl->task_capacity,
                   wl->item_capacity,
-                  alpha};
+                  alpha,
+                  c_scatter ? c_scatter->tiles : nullptr,
+                  wl->task_tile};
   hipStream_t s = tamx::as_stream(stream);
   if (a_plan->limb_type == TAMX_LIMB_I8) {
+    if (cfg && cfg->bm == SBM && cfg->bn == SBN && cfg->bk == 2 * SBK8)
+      return launch_split_gemm_i8_2w(a_plan->nslices, ctx, wl, prm, s);
     if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
       return launch_split_gemm_wide(TAMX_LIMB_I8, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
                                     prm, s);
@@ -1292,13 +1306,21 @@ int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double 
 int tamx_grouped_gemm_f32x2(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
                             double alpha, const double* A, const double* B, double* C,
                             void* stream) {
+  return tamx_grouped_gemm_f32x2_scatter(ctx, wl, op_a, op_b, alpha, A, B, C, nullptr, stream);
+}
+
+int tamx_grouped_gemm_f32x2_scatter(tamx_context* ctx, tamx_worklist* wl, int op_a, int op_b,
+                                    double alpha, const double* A, const double* B, double* C,
+                                    const tamx_gather_plan* c_scatter, void* stream) {
   TAMX_REQUIRE(wl != nullptr, "null work list");
+  TAMX_REQUIRE(c_scatter == nullptr || c_scatter->ctx == ctx, "scatter plan of another context");
   TAMX_REQUIRE(op_a == 0 || op_a == 1, "op_a must be TAMX_NO_TRANS or TAMX_TRANS");
   TAMX_REQUIRE(op_b == 0 || op_b == 1, "op_b must be TAMX_NO_TRANS or TAMX_TRANS");
   if (int rc = tamx::activate(ctx)) return rc;
   SplitParams prm{nullptr, nullptr, nullptr, nullptr, C, wl->tasks, wl->pairs, wl->counters,
                   wl->item_prefix, wl->item_task, wl->queue, wl->task_capacity,
-                  wl->item_capacity, alpha};
+                  wl->item_capacity, alpha, c_scatter ? c_scatter->tiles : nullptr,
+                  wl->task_tile};
   hipStream_t s = tamx::as_stream(stream);
   if (!op_a && !op_b) return launch_f32x2<false, false>(ctx, wl, A, B, prm, s);
   if (!op_a && op_b) return launch_f32x2<false, true>(ctx, wl, A, B, prm, s);

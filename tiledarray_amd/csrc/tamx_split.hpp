@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "tamx_internal.hpp"
+#include "tamx_map.hpp"
 
 namespace tamx {
 namespace split {
@@ -52,6 +53,10 @@ struct SplitParams {
   int64_t task_capacity;
   int64_t item_capacity;
   double alpha;
+  // result tiles stored through a scatter map (null: row-major with the
+  // task's ldc); task_tile[t] is task t's position in the result grid
+  const tamx::GatherTile* __restrict__ c_scatter;
+  const int32_t* __restrict__ task_tile;
 };
 
 constexpr int kThreads = 256;
@@ -86,6 +91,11 @@ __device__ inline int lds_unit(int r, int q) { return r * 4 + (q ^ ((r >> 1) & 3
 // for 2 to 4 limbs (tamx_split_wide.hip); plans its items and launches.
 int launch_split_gemm_wide(int limb_type, int nslices, int bm, int bn, tamx_context* ctx,
                            tamx_worklist* wl, const SplitParams& prm, hipStream_t s);
+// The grouped split GEMM for 4 or 5 INT8 limbs with two-wave workgroups,
+// 64 x 64 macro tiles, A limbs through an LDS ring and B limbs straight into
+// registers (tamx_split_i8.hip); plans its items and launches.
+int launch_split_gemm_i8_2w(int nslices, tamx_context* ctx, tamx_worklist* wl,
+                            const SplitParams& prm, hipStream_t s);
 
 }  // namespace split
 }  // namespace tamx

@@ -1,0 +1,345 @@
+// Grouped split GEMM on INT8 limbs, two-wave workgroups.
+//
+// Same arithmetic as split_gemm_kernel<S, true> (tamx_split.hip): per tile
+// pair the limb products s + t < S on the INT8 MFMA 16x16x64, one exact
+// INT32 accumulator per level s + t, folded into FP64 at pair ends with the
+// scale 2^(sigma_A + sigma_B - 14) and the level weights 2^-7(s+t).  The
+// results are identical bit for bit: every level sum is exact and the folds
+// happen at the same points in the same order.
+//
+// What differs is how the limbs reach the MFMAs.  A workgroup of two waves
+// computes a 64 x 64 macro tile, each wave 64 rows x 32 columns (4 x 2
+// blocks of 16 x 16), so every limb fragment a wave reads feeds four or
+// two blocks instead of two.  The A limbs (shared by both waves) go from
+// global memory straight into a three-stage LDS ring; the B limbs (each
+// wave its own columns) go straight into registers, three sets in a ring.
+// Chunk c + 2 is loaded while chunk c is multiplied, one barrier per chunk.
+// Per 64-k chunk a wave reads 20 KB of fragments from LDS and 10 KB from
+// global memory against 120 MFMAs (1920 cycles), half the LDS traffic per
+// MFMA of the four-wave kernel.  One wave per SIMD: the level and FP64
+// accumulators alone take 224 registers.
+#include <type_traits>
+
+#include "tamx_internal.hpp"
+#include "tamx_split.hpp"
+
+namespace {
+
+using namespace tamx::split;
+
+constexpr int kWaves = 2;
+constexpr int kT = 64 * kWaves;  // threads per workgroup
+constexpr int kStages = 3;       // A chunks in the LDS ring
+constexpr int IBM = 64, IBN = 64;
+constexpr int kAUnits = IBM * 4;          // 16-byte units of one limb plane of an A chunk
+constexpr int kALoads = kAUnits / kT;     // direct loads per thread per plane
+static_assert(kAUnits % kT == 0, "A chunk split over the threads");
+
+// vmcnt wait (the other counters left alone), gfx9 encoding
+constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+template <int S>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void split_gemm_i8_kernel(SplitParams p) {
+  constexpr int NG = S;     // one level per s + t
+  constexpr int CK = SBK8;  // 64 k per chunk
+  // vector memory operations per thread per chunk: A into LDS, B into registers
+  constexpr int kOps = S * kALoads + S * 2;
+  __shared__ i4_t sA[kStages][S][kAUnits];
+  __shared__ int s_item;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = uni(tid >> 6);
+  const int wc = wave * 32;                    // the wave's columns
+  const int lrow = lane & 15, lq = lane >> 4;  // MFMA: row lrow, k unit lq
+  const auto* LA = reinterpret_cast<const i4_t*>(p.LA);
+  const auto* LB = reinterpret_cast<const i4_t*>(p.LB);
+
+  int64_t ntasks64 = p.counters[TAMX_WL_NTASKS];
+  if (ntasks64 < 0 || ntasks64 > p.task_capacity || p.counters[TAMX_WL_STATUS] != 0)
+    ntasks64 = 0;
+  const int32_t ntasks = uni((int32_t)ntasks64);
+  const int32_t total = uni(p.prefix[ntasks > 0 ? ntasks : 0]);
+  const int home = blockIdx.x % kRegions;
+  int tried = 0;
+  for (;;) {
+    __syncthreads();  // both waves are done with the LDS ring and s_item
+    if (tid == 0) {
+      int it = -1;
+      while (tried < kRegions) {
+        const int r = (home + tried) % kRegions;
+        const int rb = (int)((int64_t)total * r / kRegions);
+        const int re = (int)((int64_t)total * (r + 1) / kRegions);
+        const int q = rb < re ? atomicAdd(&p.queue[r], 1) : 0;
+        if (rb + q < re) {
+          it = rb + q;
+          break;
+        }
+        ++tried;
+      }
+      s_item = it;
+    }
+    __syncthreads();
+    const int item = uni(s_item);
+    if (item < 0) break;
+    int lo;
+    if (total <= p.item_capacity) {
+      lo = uni(p.item_task[item]);
+    } else {
+      lo = 0;
+      int hi = ntasks - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uni(p.prefix[mid]) <= item) lo = mid; else hi = mid - 1;
+      }
+    }
+    const auto* tp = tamx::scalar_ptr(p.tasks) + lo;
+    const int64_t t_coff = uni64(tp->c_off);
+    const int t_m = uni(tp->m), t_n = uni(tp->n), t_ldc = uni(tp->ldc);
+    const int t_pb = uni(tp->pair_begin), t_pc = uni(tp->pair_count);
+    const int t_flags = uni(tp->flags);
+    const int local = item - uni(p.prefix[lo]);
+    const int tiles_n = (t_n + IBN - 1) / IBN;
+    const int r0 = (local / tiles_n) * IBM, c0 = (local % tiles_n) * IBN;
+    const int mrem = min(IBM, t_m - r0), nrem = min(IBN, t_n - c0);
+    // 16-row blocks inside the tile (both waves) and the wave's 16-column
+    // blocks inside it; a wave with none skips its MFMAs
+    const int ni = uni(min(4, (mrem + 15) / 16));
+    const int nj = uni(max(0, min(2, (nrem - wc + 15) / 16)));
+    const bool wave_active = nj > 0;
+
+    d4_t acc[4][2];
+    i4_t lvl[NG][4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
+      }
+
+    const int pend = t_pb + t_pc;
+
+    // load cursor: pair lp, chunk lk0, two chunks ahead of compute.  Per
+    // pair: this thread's A units (16-byte units of the limb pool) of its
+    // kALoads slots in plane 0 at k = 0, the plane strides, and its B
+    // fragment units (rows c0 + wc + 16 j + lrow, k unit lq).
+    int lp = t_pb, lk0 = 0, lkp = 0;
+    int64_t a_unit[kALoads], b_unit[2], a_plane = 0, b_plane = 0;
+    auto set_pair = [&](int pi) {
+      const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
+      const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
+      const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
+      lkp = uni(A->k_pad);
+      const int64_t kp16 = lkp >> 4;
+      a_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> 4;
+      b_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> 4;
+      const int64_t a0 = uni64(A->limb_off) >> 4, b0 = uni64(B->limb_off) >> 4;
+#pragma unroll
+      for (int h = 0; h < kALoads; ++h) {
+        // LDS slot L of this lane in load h holds row L / 4, unit L % 4 of
+        // the swizzled layout, i.e. source unit q = (L % 4) ^ ((row / 2) % 4)
+        const int L = kT * h + 64 * wave + lane;
+        const int r = L >> 2, q = (L & 3) ^ ((r >> 1) & 3);
+        a_unit[h] = a0 + (int64_t)(r0 + r) * kp16 + q;
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b_unit[j] = b0 + (int64_t)(c0 + wc + 16 * j + lrow) * kp16 + lq;
+    };
+    if (lp < pend) set_pair(lp);
+    // Loads of the cursor's chunk: A into LDS stage `st`, B into `bs`.
+    // Once the cursor has passed the last pair the loads repeat a chunk of
+    // the last pair, unused, so every step issues the same number of loads
+    // and the wait counts stay exact.
+    auto load_chunk = [&](int st, i4_t (&bs)[S][2]) {
+      const int64_t k16 = lk0 >> 4;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int h = 0; h < kALoads; ++h)
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(LA + a_unit[h] + s * a_plane + k16),
+              (__attribute__((address_space(3))) void*)(&sA[st][s][kT * h + 64 * wave]), 16, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bs[s][j] = *tamx::global_ptr(LB + b_unit[j] + s * b_plane + k16);
+      }
+      if (lp < pend) {
+        lk0 += CK;
+        if (lk0 >= lkp) {
+          ++lp;
+          lk0 = 0;
+          if (lp < pend) set_pair(lp);
+        }
+      }
+    };
+
+    // The wave's 16-row blocks i < ni and 16-column blocks j < nj lie inside
+    // the tile (uniform per wave).  Block rows are multiplied one after the
+    // other, each behind its own uniform branch, so a ragged edge costs no
+    // MFMAs; the next block row's A fragments are read from LDS (always,
+    // rows past the edge hold whatever lies there) while the current one's
+    // MFMAs run.
+    auto compute_chunk = [&](int st, const i4_t (&bs)[S][2]) {
+      if (!wave_active) return;
+      i4_t fa[2][S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) fa[0][s] = sA[st][s][lds_unit(lrow, lq)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int cur = i & 1;
+        if (i < 3) {
+#pragma unroll
+          for (int s = 0; s < S; ++s) fa[cur ^ 1][s] = sA[st][s][lds_unit((i + 1) * 16 + lrow, lq)];
+        }
+        if (i < ni) {
+          if (nj == 2) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+              for (int t = 0; t < S - s; ++t)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                  lvl[s + t][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                      fa[cur][s], bs[t][j], lvl[s + t][i][j], 0, 0, 0);
+          } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+              for (int t = 0; t < S - s; ++t)
+                lvl[s + t][i][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                    fa[cur][s], bs[t][0], lvl[s + t][i][0], 0, 0, 0);
+          }
+        }
+      }
+    };
+    // fold the levels of the pair just finished into FP64 (level weights
+    // 2^-7g are exact, so each fma rounds once, as a sum would); blocks past
+    // the edge hold zeros and are skipped
+    auto fold = [&](double scale) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i < ni) {
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              double tsum = 0.0;
+#pragma unroll
+              for (int g = NG - 1; g >= 0; --g) {
+                tsum = fma((double)lvl[g][i][j][v], 1.0 / (double)(1LL << (7 * g)), tsum);
+                lvl[g][i][j][v] = 0;
+              }
+              acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
+            }
+        }
+      }
+    };
+
+    // compute cursor: pair cp, chunk ck0; chunks since the last fold
+    int cp = t_pb, ck0 = 0, ckp = 0, since_fold = 0;
+    double cscale = 0.0;
+    auto set_compute_pair = [&](int pi) {
+      const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
+      const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
+      const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
+      ckp = uni(A->k_pad);
+      cscale = ldexp(1.0, uni(A->sigma) + uni(B->sigma) - 14);
+    };
+    if (cp < pend) set_compute_pair(cp);
+    auto after_compute = [&]() {
+      ++since_fold;
+      ck0 += CK;
+      const bool pair_end = ck0 >= ckp;
+      if (pair_end || since_fold == kFoldChunks8) {
+        if (wave_active) fold(cscale);
+        since_fold = 0;
+      }
+      if (pair_end) {
+        ++cp;
+        ck0 = 0;
+        if (cp < pend) set_compute_pair(cp);
+      }
+    };
+
+    i4_t b0[S][2], b1[S][2], b2[S][2];
+    if (lp < pend) {
+      load_chunk(0, b0);  // chunk 0
+      load_chunk(1, b1);  // chunk 1 (or a repeat)
+      // one step: chunk c in stage X with B in bx; chunk c + 1 in flight;
+      // chunk c + 2 goes to stage (X + 2) % 3, which held chunk c - 1
+      auto step = [&](int X, const i4_t (&bx)[S][2], i4_t (&bn)[S][2]) {
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(kOps));
+        __syncthreads();
+        load_chunk(X == 0 ? 2 : X - 1, bn);
+        compute_chunk(X, bx);
+        after_compute();
+        return cp < pend;
+      };
+      for (;;) {
+        if (!step(0, b0, b2)) break;
+        if (!step(1, b1, b0)) break;
+        if (!step(2, b2, b1)) break;
+      }
+      // the repeats still in flight land in the ring before it is reused
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+
+    const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
+    double* __restrict__ c = p.C + t_coff;
+    tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            // 16x16 MFMA output: row 4 * (lane / 16) + v, column lane % 16
+            const int row = i * 16 + lq * 4 + v;
+            const int col = wc + j * 16 + lrow;
+            if (row < mrem && col < nrem) {
+              const int64_t idx = cidx(r0 + row, c0 + col);
+              double val = p.alpha * acc[i][j][v];
+              if (beta1) val += c[idx];
+              c[idx] = val;
+            }
+          }
+    });
+  }
+
+  if (tid == 0) {
+    __threadfence();
+    const int done = atomicAdd(&p.queue[kRegions], 1);
+    if (done == (int)gridDim.x - 1) {
+      for (int r = 0; r <= kRegions; ++r) atomicExch(&p.queue[r], 0);
+    }
+  }
+}
+
+template <int S>
+int launch_i8(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm, hipStream_t s) {
+  auto kern = split_gemm_i8_kernel<S>;
+  int occ = 0;
+  TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kT, 0));
+  if (occ < 1) occ = 1;
+  if (int rc = tamx::plan_items(wl, IBM, IBN, s)) return rc;
+  kern<<<ctx->num_cus * occ, kT, 0, s>>>(prm);
+  TAMX_LAUNCH_CHECK();
+  return TAMX_OK;
+}
+
+}  // namespace
+
+int tamx::split::launch_split_gemm_i8_2w(int nslices, tamx_context* ctx, tamx_worklist* wl,
+                                         const SplitParams& prm, hipStream_t s) {
+  switch (nslices) {
+    case 4: return launch_i8<4>(ctx, wl, prm, s);
+    case 5: return launch_i8<5>(ctx, wl, prm, s);
+    default: break;
+  }
+  tamx::set_error("the two-wave INT8 split GEMM takes 4 or 5 limbs");
+  return TAMX_ERR_UNSUPPORTED;
+}

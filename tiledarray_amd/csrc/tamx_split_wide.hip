@@ -275,25 +275,25 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
     }
 
     const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
-    const bool trans = (t_flags & TAMX_TASK_STORE_TRANS) != 0;
     double* __restrict__ c = p.C + t_coff;
+    tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
 #pragma unroll
-    for (int i = 0; i < WI; ++i)
+      for (int i = 0; i < WI; ++i)
 #pragma unroll
-      for (int j = 0; j < WJ; ++j)
+        for (int j = 0; j < WJ; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          // f32-output 16x16 MFMA layout: row 4 * (lane / 16) + v, column lane % 16
-          const int row = wr + i * 16 + lq * 4 + v;
-          const int col = wc + j * 16 + lrow;
-          if (row < mrem && col < nrem) {
-            const int64_t idx = trans ? (int64_t)(c0 + col) * t_ldc + r0 + row
-                                      : (int64_t)(r0 + row) * t_ldc + c0 + col;
-            double val = p.alpha * acc[i][j][v];
-            if (beta1) val += c[idx];
-            c[idx] = val;
+          for (int v = 0; v < 4; ++v) {
+            // f32-output 16x16 MFMA layout: row 4 * (lane / 16) + v, column lane % 16
+            const int row = wr + i * 16 + lq * 4 + v;
+            const int col = wc + j * 16 + lrow;
+            if (row < mrem && col < nrem) {
+              const int64_t idx = cidx(r0 + row, c0 + col);
+              double val = p.alpha * acc[i][j][v];
+              if (beta1) val += c[idx];
+              c[idx] = val;
+            }
           }
-        }
+    });
   }
 
   if (tid == 0) {

@@ -379,11 +379,14 @@ class Contraction:
     "f32x2" (two FP32 limbs).  fuse_permute: how a `general` operand is
     brought into GEMM layout (None: an index map for the limb precisions, a
     permuted copy otherwise; True: an index map; False: a copy).
+    fuse_result_permute: a general result order is written by the GEMM
+    through each result tile's index map (True) or by a permute pass over
+    the finished tiles (False).
     """
 
     def __init__(self, ctx, target, A, left, B, right, alpha=1.0, threshold=FLOAT_EPS,
                  pair_threshold=0.0, precision="f64", tile=None, permute_operands=True,
-                 fuse_permute=None):
+                 fuse_permute=None, fuse_result_permute=True):
         self.ctx = ctx
         self.alpha = float(alpha)
         self.precision = precision
@@ -449,6 +452,16 @@ class Contraction:
         self.dtype = A.pool.dtype
         self.C = torch.empty(self.c_capacity, dtype=self.dtype, device=dev)
         self.store_trans = bool(plan.result_perm_matrix_transpose)
+        # a general result permutation (contract_reduce.h:370-378) is fused
+        # into the GEMM's store: every result tile is written straight into
+        # the target order through its index map, so no permute pass runs
+        # and result() needs nothing from the host
+        self.c_scatter = None
+        if fuse_result_permute and not plan.result_perm_identity and not self.store_trans:
+            maps = self._scatter_maps(l_outer_r, r_outer_r,
+                                      [int(plan.result_perm[d]) for d in range(plan.result_rank)])
+            if maps is not None:
+                self.c_scatter = GatherPlan(ctx, maps)
         grid = self.Mt * self.Nt
         self.wl = WorkList(ctx, grid, max(1, grid * self.Kt), grid)
         self.pair_threshold = float(pair_threshold)
@@ -518,6 +531,43 @@ class Contraction:
              nt, offs.ctypes.data, nr, nk, rext.ctypes.data, rstr.ctypes.data,
              kext.ctypes.data, kstr.ctypes.data, ctypes.byref(p))
         return _SplitOperand(p, self._limb_pool(p, X), nt)
+
+    @staticmethod
+    def _scatter_maps(l_outer, r_outer, rp):
+        """Index maps of the natural result tiles (left-outer modes as rows,
+        right-outer modes as columns) into the stored target-order tiles:
+        natural mode d lands at target position rp[d] ("TO" convention), so
+        its stride is the product of the target extents after rp[d].  None
+        when a side has more than four modes (the result is then permuted
+        after the GEMM)."""
+        nl, nr = len(l_outer), len(r_outer)
+        if not (1 <= nl <= 4 and 1 <= nr <= 4):
+            return None
+        ranges = list(l_outer) + list(r_outer)
+        rank = nl + nr
+        Mt = int(np.prod([r.ntiles for r in l_outer], dtype=np.int64))
+        Nt = int(np.prod([r.ntiles for r in r_outer], dtype=np.int64))
+        nt = Mt * Nt
+        rext = np.ones((nt, nl), np.int32)
+        rstr = np.zeros((nt, nl), np.int64)
+        kext = np.ones((nt, nr), np.int32)
+        kstr = np.zeros((nt, nr), np.int64)
+        lidx = np.array(np.unravel_index(np.arange(Mt), [r.ntiles for r in l_outer])).T
+        ridx = np.array(np.unravel_index(np.arange(Nt), [r.ntiles for r in r_outer])).T
+        for i in range(Mt):
+            for j in range(Nt):
+                t = i * Nt + j
+                e = ([int(l_outer[d].extents[lidx[i, d]]) for d in range(nl)] +
+                     [int(r_outer[d].extents[ridx[j, d]]) for d in range(nr)])
+                tgt = [0] * rank
+                for d in range(rank):
+                    tgt[rp[d]] = e[d]
+                tstride = [int(np.prod(tgt[q + 1:], dtype=np.int64)) for q in range(rank)]
+                rext[t] = e[:nl]
+                kext[t] = e[nl:]
+                rstr[t] = [tstride[rp[d]] for d in range(nl)]
+                kstr[t] = [tstride[rp[d]] for d in range(nl, rank)]
+        return nt, nl, nr, rext, rstr, kext, kstr
 
     def _gemm_modes(self, is_right):
         plan = self.plan
@@ -685,7 +735,23 @@ class Contraction:
         s = self.ctx.stream()
         A, B, C = self.A.pool.data_ptr(), self.B.pool.data_ptr(), self.C.data_ptr()
         cfg = None if self.tile is None else ctypes.byref(self.tile_cfg)
-        if self.a_gather is not None or self.b_gather is not None:
+        if self.c_scatter is not None:
+            sc = self.c_scatter.ptr
+            if self.precision in ("f64", "f32"):
+                f64 = self.precision == "f64"
+                call("tamx_grouped_gemm_scatter_f64" if f64 else "tamx_grouped_gemm_scatter_f32",
+                     self.ctx.ptr, self.wl.ptr, self.gemm_op_a, self.gemm_op_b,
+                     ctypes.c_double(self.alpha) if f64 else ctypes.c_float(self.alpha),
+                     A, _plan_ptr(self.a_gather), B, _plan_ptr(self.b_gather), C, sc, cfg, s)
+            elif self.precision == "f32x2":
+                call("tamx_grouped_gemm_f32x2_scatter", self.ctx.ptr, self.wl.ptr, self.op_a,
+                     self.op_b, ctypes.c_double(self.alpha), A, B, C, sc, s)
+            else:
+                call("tamx_grouped_gemm_split_scatter", self.ctx.ptr, self.wl.ptr,
+                     ctypes.c_double(self.alpha), self.a_split.ptr,
+                     self.a_split.limbs.data_ptr(), self.b_split.ptr,
+                     self.b_split.limbs.data_ptr(), C, sc, cfg, s)
+        elif self.a_gather is not None or self.b_gather is not None:
             f64 = self.precision == "f64"
             call("tamx_grouped_gemm_gather_f64" if f64 else "tamx_grouped_gemm_gather_f32",
                  self.ctx.ptr, self.wl.ptr, self.gemm_op_a, self.gemm_op_b,
@@ -725,7 +791,11 @@ class Contraction:
                 "celems": c[_lib.WL_CELEMS], "flops": c[_lib.WL_FLOPS]}
 
     def result(self):
-        """The result DistArray in target order (synchronises)."""
+        """The result DistArray in target order.  Its tiles stay in HBM and
+        nothing is read back: identity, transposed and (up to four modes a
+        side) general result orders are written by the GEMM itself; only a
+        general order with more modes a side runs a permute pass, whose
+        plan needs the result offsets on the host."""
         plan = self.plan
         nat = TiledRange(self.natural_ranges)
         rp = [int(plan.result_perm[d]) for d in range(plan.result_rank)]
@@ -738,7 +808,7 @@ class Contraction:
         inv = [int(i) for i in np.argsort(rp)]
         grid_perm = lambda t: t.view(nat.tiles_shape).permute(*inv).contiguous().view(-1)
         tnorms = grid_perm(norms) if norms is not None else None
-        if self.store_trans:
+        if self.store_trans or self.c_scatter is not None:
             return DistArray(tgt, self.C, d_offsets=grid_perm(self.c_off), shape_norms=tnorms,
                              shape_threshold=thr)
         # general result permutation: permute every non-zero tile in place
