@@ -182,6 +182,19 @@ typedef struct tamx_norm_plan tamx_norm_plan;
 int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
                           const int64_t* host_volumes, int64_t ntiles,
                           tamx_norm_plan** plan);
+/* A plan over the outer tiles of a tensor of tensors: tile t is the cells
+ * [host_tile_cells[t], host_tile_cells[t+1]) of the cell table, cell c at
+ * element offset host_cell_off[c] (< 0: empty cell) with host_cell_vol[c]
+ * elements, wherever they lie in the pool.  The norm of tile t is
+ * sqrt(sum over its cells, in order, of each cell's sum of squares), as
+ * Tensor<Tensor<T>>::squared_norm reduces its cells
+ * (tensor/tensor.h:3768-3776), cast to float.  Used with
+ * tamx_tile_norms_f64 / _f32 like any norm plan. */
+int tamx_norm_plan_create_cells(tamx_context* ctx, int64_t ntiles,
+                                const int64_t* host_tile_cells,
+                                const int64_t* host_cell_off,
+                                const int64_t* host_cell_vol,
+                                tamx_norm_plan** plan);
 int tamx_norm_plan_destroy(tamx_norm_plan* plan);
 int tamx_tile_norms_f64(tamx_context* ctx, const tamx_norm_plan* plan,
                         const double* pool, float* norms, void* stream);
@@ -489,8 +502,12 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             double* C, void* stream);
 /* The same with a macro tile: bm x bn of 64 x 64 (the default, any limb
  * count), 128 x 64 or 64 x 128 (2 to 4 BF16 or 3 to 4 INT8 limbs; one wave
- * per SIMD, a quarter fewer limb bytes loaded per result).  bk and
- * wgs_per_cu are ignored. */
+ * per SIMD, a quarter fewer limb bytes loaded per result).  For INT8 limbs
+ * at 64 x 64, bk picks the kernel: 64, each wave's 32 x 32 quadrant on the
+ * INT8 MFMA 32x32x32; 128 (4 or 5 limbs), two-wave workgroups with A limbs
+ * through an LDS ring and B limbs straight into registers; otherwise the
+ * 16x16x64 MFMA.  All give the same result bit for bit.  wgs_per_cu is
+ * ignored. */
 int tamx_grouped_gemm_split_config(tamx_context* ctx, tamx_worklist* wl, double alpha,
                                    const tamx_split_plan* a_plan, const void* a_limbs,
                                    const tamx_split_plan* b_plan, const void* b_limbs,

@@ -40,6 +40,8 @@ def lib():
         L.ora_norm_f32.argtypes = [_vp, ctypes.c_int64]
         for n in ("ora_tile_norms_f64", "ora_tile_norms_f32"):
             getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp]
+        for n in ("ora_tot_tile_norms_f64", "ora_tot_tile_norms_f32"):
+            getattr(L, n).argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp]
         L.ora_scale_tile_norms.restype = ctypes.c_int64
         L.ora_scale_tile_norms.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_float]
         L.ora_shape_gemm.restype = ctypes.c_int64
@@ -99,6 +101,20 @@ def tile_norms(pool, offsets, volumes):
     out = np.zeros(offsets.size, np.float32)
     f = lib().ora_tile_norms_f64 if pool.dtype == np.float64 else lib().ora_tile_norms_f32
     f(_p(pool), _p(offsets), _p(volumes), offsets.size, _p(out))
+    return out
+
+
+def tot_tile_norms(pool, tile_cells, cell_off, cell_vol):
+    """Outer-tile norms of a tensor of tensors (cells reduced in order,
+    each cell's own sequential sum added; tensor.h:3768-3776)."""
+    pool = np.ascontiguousarray(pool)
+    tile_cells = _c(tile_cells, np.int64)
+    cell_off = _c(cell_off, np.int64)
+    cell_vol = _c(cell_vol, np.int64)
+    nt = tile_cells.size - 1
+    out = np.zeros(max(nt, 0), np.float32)
+    f = lib().ora_tot_tile_norms_f64 if pool.dtype == np.float64 else lib().ora_tot_tile_norms_f32
+    f(_p(pool), _p(tile_cells), _p(cell_off), _p(cell_vol), nt, _p(out))
     return out
 
 

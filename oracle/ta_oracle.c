@@ -135,6 +135,45 @@ void ora_tile_norms_f32(const float* pool, const int64_t* offsets,
     out[t] = offsets[t] < 0 ? 0.0f : ora_norm_f32(pool + offsets[t], volumes[t]);
 }
 
+/* Tensor<Tensor<T>>::squared_norm (tensor.h:3768-3776): the outer reduce
+ * walks the cells in order and adds each cell's own squared norm (an inner
+ * sequential reduce, as above) to a scalar_type accumulator; the outer
+ * tile's norm is its sqrt, stored as float.  Tile t holds cells
+ * [tile_cells[t], tile_cells[t+1]) of the cell table (offset < 0: empty). */
+void ora_tot_tile_norms_f64(const double* pool, const int64_t* tile_cells,
+                            const int64_t* cell_off, const int64_t* cell_vol,
+                            int64_t ntiles, float* out) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(ora_num_threads())
+  for (int64_t t = 0; t < ntiles; ++t) {
+    double res = 0.0;
+    for (int64_t c = tile_cells[t]; c < tile_cells[t + 1]; ++c) {
+      if (cell_off[c] < 0) continue;
+      double cs = 0.0;
+      const double* x = pool + cell_off[c];
+      for (int64_t i = 0; i < cell_vol[c]; ++i) cs += x[i] * x[i];
+      res += cs;
+    }
+    out[t] = (float)sqrt(res);
+  }
+}
+
+void ora_tot_tile_norms_f32(const float* pool, const int64_t* tile_cells,
+                            const int64_t* cell_off, const int64_t* cell_vol,
+                            int64_t ntiles, float* out) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(ora_num_threads())
+  for (int64_t t = 0; t < ntiles; ++t) {
+    float res = 0.0f;
+    for (int64_t c = tile_cells[t]; c < tile_cells[t + 1]; ++c) {
+      if (cell_off[c] < 0) continue;
+      float cs = 0.0f;
+      const float* x = pool + cell_off[c];
+      for (int64_t i = 0; i < cell_vol[c]; ++i) cs += x[i] * x[i];
+      res += cs;
+    }
+    out[t] = sqrtf(res);
+  }
+}
+
 /* ----------------------------------------------------- shape scaling ---- */
 
 /* recursive_outer_product (sparse_shape.h:106-131) with the inverse-volume op

@@ -35,6 +35,18 @@ void ora_tile_norms_f64(const double* pool, const int64_t* offsets,
 void ora_tile_norms_f32(const float* pool, const int64_t* offsets,
                         const int64_t* volumes, int64_t ntiles, float* out);
 
+/* Norms of the outer tiles of a tensor of tensors
+ * (Tensor<Tensor<T>>::squared_norm, tensor/tensor.h:3768-3776): per tile,
+ * the cells [tile_cells[t], tile_cells[t+1]) in order, each cell's
+ * sequential sum of squares added to the tile's accumulator; empty cells
+ * (cell_off < 0) skipped. */
+void ora_tot_tile_norms_f64(const double* pool, const int64_t* tile_cells,
+                            const int64_t* cell_off, const int64_t* cell_vol,
+                            int64_t ntiles, float* out);
+void ora_tot_tile_norms_f32(const float* pool, const int64_t* tile_cells,
+                            const int64_t* cell_off, const int64_t* cell_vol,
+                            int64_t ntiles, float* out);
+
 /* SparseShape::scale_tile_norms<InverseVolume, Screen=true>
  * (sparse_shape.h:150-217, recursive_outer_product :106-131).
  * norms: row-major tile-norm tensor of the given rank, scaled in place.

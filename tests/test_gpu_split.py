@@ -125,11 +125,13 @@ def test_split_wide_tiles_match_default(ctx, tile, precision):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
+@pytest.mark.parametrize("bk", [64, 128])
 @pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
-def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b):
-    # the two-wave INT8 kernel (64 x 64 macro tiles, bk 128 selects it) on
-    # tiles ragged in both directions, narrower and wider than a macro tile
+def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b, bk):
+    # the INT8 kernels on the 32x32x32 MFMA (64 x 64 macro tiles; bk 64:
+    # four waves, bk 128: two waves) on tiles ragged in both directions,
+    # narrower and wider than a macro tile
     nslices = int(precision.split("x")[-1])
     rng = np.random.default_rng(1300 + nslices * 4 + op_a * 2 + op_b)
     trm = TiledRange1([0, 40, 170, 300, 301])
@@ -139,7 +141,7 @@ def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b):
     la = "k,m" if op_a else "m,k"
     lb = "n,k" if op_b else "k,n"
     c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=precision,
-                      tile=(64, 64, 128))
+                      tile=(64, 64, bk))
     c.run()
     torch.cuda.synchronize()
     Cref, npairs, c_off = oracle_contract(c, A, B)
@@ -147,10 +149,11 @@ def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b):
     np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
 
 
+@pytest.mark.parametrize("bk", [64, 128])
 @pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
-def test_split_i8_two_wave_matches_default(ctx, precision):
-    # same limbs, same pair order, same folds: the two-wave kernel's result
-    # equals the four-wave kernel's bit for bit on random block-sparse data,
+def test_split_i8_two_wave_matches_default(ctx, precision, bk):
+    # same limbs, same pair order, same folds: the 32x32x32 kernels' results
+    # equal the 16x16x64 kernel's bit for bit on random block-sparse data,
     # and on a second run over a smaller problem after it
     for n, seed in ((1500, 31), (700, 32)):
         A, B, _ = W.blocksparse_gemm(n=n, wlo=24, whi=150, seed=seed)
@@ -161,7 +164,7 @@ def test_split_i8_two_wave_matches_default(ctx, precision):
         c.run()
         torch.cuda.synchronize()
         C0 = c.C.clone()
-        c.set_tile((64, 64, 128))
+        c.set_tile((64, 64, bk))
         c.C.zero_()
         c.gemm_step()
         torch.cuda.synchronize()

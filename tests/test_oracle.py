@@ -335,3 +335,25 @@ def test_permopt_inner_order_mismatch_respects_preference():
     assert lt == permopt.IDENTITY and rt == permopt.GENERAL
     l, r, res, lt, rt = permopt.compute_index_list_contraction(list("kl"), list("lkj"), False)
     assert lt == permopt.GENERAL and rt == permopt.IDENTITY
+
+
+def test_tot_tile_norms_closed_forms_and_cell_order():
+    # outer-tile norms of a tensor of tensors (Tensor<Tensor>::squared_norm,
+    # tensor.h:3768-3776): all-ones cells give sqrt(total elements), empty
+    # cells are skipped, a one-cell tile equals the plain tile norm
+    pool = np.ones(100)
+    tc = np.array([0, 3, 5, 5])
+    off = np.array([0, -1, 10, 40, 70])
+    vol = np.array([10, 99, 30, 30, 30])
+    got = oracle.tot_tile_norms(pool, tc, off, vol)
+    np.testing.assert_allclose(got, np.float32([np.sqrt(40), np.sqrt(60), 0.0]), rtol=1e-7)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, 77)
+    assert oracle.tot_tile_norms(x, [0, 1], [0], [77])[0] == oracle.tile_norms(x, [0], [77])[0]
+    # each cell is summed on its own and then added: in float, a 4096 cell
+    # followed by a cell of three ones gives 2^24 + 3 (nested), not 2^24
+    # (element by element, each 1 lost below the 2^24 spacing)
+    pool32 = np.array([4096.0, 1.0, 1.0, 1.0], np.float32)
+    n = oracle.tot_tile_norms(pool32, [0, 2], [0, 1], [1, 3])[0]
+    nested = np.float32(np.float32(4096.0) * np.float32(4096.0)) + np.float32(3.0)
+    assert n == np.sqrt(nested, dtype=np.float32) and n != np.float32(4096.0)

@@ -109,6 +109,7 @@ SIGNATURES = {
     "tamx_plan_contraction": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i,
                               ctypes.POINTER(ContractionPlan)],
     "tamx_norm_plan_create": [_vp, _vp, _vp, _i64, _vp],
+    "tamx_norm_plan_create_cells": [_vp, _i64, _vp, _vp, _vp, _vp],
     "tamx_norm_plan_destroy": [_vp],
     "tamx_tile_norms_f64": [_vp, _vp, _vp, _vp, _vp],
     "tamx_tile_norms_f32": [_vp, _vp, _vp, _vp, _vp],

@@ -648,6 +648,68 @@ int tamx_norm_plan_create(tamx_context* ctx, const int64_t* host_offsets,
   return TAMX_OK;
 }
 
+int tamx_norm_plan_create_cells(tamx_context* ctx, int64_t ntiles, const int64_t* host_tile_cells,
+                                const int64_t* host_cell_off, const int64_t* host_cell_vol,
+                                tamx_norm_plan** out) {
+  TAMX_REQUIRE(out != nullptr, "null output pointer");
+  *out = nullptr;
+  TAMX_REQUIRE(ntiles >= 0, "negative tile count");
+  TAMX_REQUIRE(ntiles == 0 || host_tile_cells, "null tile table");
+  if (int rc = tamx::activate(ctx)) return rc;
+  // the chunks of every cell of tile t, cell by cell: the finish sums them
+  // in this order
+  std::vector<int64_t> cbeg, cend, tchunk(ntiles + 1);
+  std::vector<int32_t> ctile;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    tchunk[t] = (int64_t)cbeg.size();
+    const int64_t c0 = host_tile_cells[t], c1 = host_tile_cells[t + 1];
+    TAMX_REQUIRE(c0 >= 0 && c1 >= c0, "tile cell ranges must be a non-decreasing prefix");
+    TAMX_REQUIRE(c1 == c0 || (host_cell_off && host_cell_vol), "null cell table");
+    for (int64_t c = c0; c < c1; ++c) {
+      if (host_cell_off[c] < 0) continue;
+      TAMX_REQUIRE(host_cell_vol[c] >= 0, "negative cell volume");
+      for (int64_t b = 0; b < host_cell_vol[c]; b += kNormChunk) {
+        cbeg.push_back(host_cell_off[c] + b);
+        cend.push_back(host_cell_off[c] + std::min(host_cell_vol[c], b + kNormChunk));
+        ctile.push_back((int32_t)t);
+      }
+    }
+  }
+  tchunk[ntiles] = (int64_t)cbeg.size();
+  auto* plan = new (std::nothrow) tamx_norm_plan();
+  if (!plan) return TAMX_ERR_OOM;
+  plan->ctx = ctx;
+  plan->ntiles = ntiles;
+  plan->nchunks = (int64_t)cbeg.size();
+  const size_t nc = std::max<size_t>(1, cbeg.size());
+  hipError_t e = hipSuccess;
+  auto alloc = [&](auto** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  };
+  alloc(&plan->cbeg, nc * sizeof(int64_t));
+  alloc(&plan->cend, nc * sizeof(int64_t));
+  alloc(&plan->ctile, nc * sizeof(int32_t));
+  alloc(&plan->partial, nc * sizeof(double));
+  alloc(&plan->partial_max, nc * sizeof(double));
+  alloc(&plan->tchunk, (ntiles + 1) * sizeof(int64_t));
+  if (e == hipSuccess && !cbeg.empty()) {
+    e = hipMemcpy(plan->cbeg, cbeg.data(), cbeg.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(plan->cend, cend.data(), cend.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(plan->ctile, ctile.data(), ctile.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->tchunk, tchunk.data(), tchunk.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tamx::set_error(std::string("norm plan: ") + hipGetErrorString(e));
+    tamx_norm_plan_destroy(plan);
+    return e == hipErrorOutOfMemory ? TAMX_ERR_OOM : TAMX_ERR_DEVICE;
+  }
+  *out = plan;
+  return TAMX_OK;
+}
+
 int tamx_norm_plan_destroy(tamx_norm_plan* plan) {
   if (!plan) return TAMX_OK;
   void* ptrs[] = {plan->cbeg, plan->cend, plan->ctile, plan->partial, plan->partial_max,

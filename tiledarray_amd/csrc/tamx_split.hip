@@ -364,8 +364,13 @@ void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_
 // INT32 accumulator per level s + t; otherwise BF16 limbs as above.  Both
 // stage 16-byte units (8 BF16 or 16 INT8 limbs of one row), 4 units of a
 // row per chunk, so the staging and the LDS layout are shared.
-template <int S, bool I8>
+// M32 (INT8 only): each wave's 32 x 32 quadrant is one block of the INT8
+// MFMA 32x32x32 instead of four of the 16x16x64 (the same products and
+// levels; on this part the 32x32x32 does twice the work in about the same
+// issue time).
+template <int S, bool I8, bool M32 = false>
 __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitParams p) {
+  static_assert(!M32 || I8, "the 32x32x32 blocks are INT8 only");
   constexpr int NG = I8 ? S : (S >= 3 ? 3 : S);  // BF16: P_0, P_1, P_T; INT8: levels
   constexpr int CK = I8 ? SBK8 : SBK;            // k per chunk
   constexpr int USH = I8 ? 4 : 3;                // log2 limbs per 16-byte unit
@@ -434,19 +439,32 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     const int mrem = min(SBM, t_m - r0), nrem = min(SBN, t_n - c0);
     const bool wave_active = (wr < mrem) && (wc < nrem);
 
-    d4_t acc[2][2];
-    lvl_t lvl[NG][2][2];
+    // 16 x 16 blocks (acc, lvl) or one 32 x 32 block (acc32, lvl32)
+    constexpr int NB = M32 ? 1 : 2;
+    d4_t acc[NB][NB];
+    lvl_t lvl[M32 ? 1 : NG][NB][NB];
+    double acc32[M32 ? 16 : 1];
+    i16_t lvl32[M32 ? NG : 1];
+    if constexpr (M32) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int v = 0; v < 16; ++v) {
+        acc32[v] = 0.0;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
+        for (int g = 0; g < NG; ++g) lvl32[g][v] = 0;
       }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+#pragma unroll
+          for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
+        }
+    }
 
     const int pend = t_pb + t_pc;
 
@@ -556,6 +574,28 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     using two = std::integral_constant<int, 2>;
     auto compute_chunk = [&](int buf) {
       if (!wave_active) return;
+      if constexpr (M32) {
+        // the quadrant as one 32 x 32 block; two 32-k steps per chunk,
+        // lane l: row (col) l % 32, k unit 2 h + l / 32
+        (void)compute_blocks;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          i4_t bf[S];
+#pragma unroll
+          for (int t = 0; t < S; ++t)
+            bf[t] = __builtin_bit_cast(
+                i4_t, sB[buf][t][lds_unit(wc + (lane & 31), 2 * h + (lane >> 5))]);
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const i4_t af = __builtin_bit_cast(
+                i4_t, sA[buf][s][lds_unit(wr + (lane & 31), 2 * h + (lane >> 5))]);
+#pragma unroll
+            for (int t = 0; t < S - s; ++t)
+              lvl32[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[t], lvl32[s + t], 0, 0, 0);
+          }
+        }
+        return;
+      }
       if (ni == 2 && nj == 2) compute_blocks(buf, two{}, two{});
       else if (ni == 2) compute_blocks(buf, two{}, one{});
       else if (nj == 2) compute_blocks(buf, one{}, two{});
@@ -565,6 +605,19 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     // 2^(sigma_A + sigma_B - 16), BF16; 2^(sigma_A + sigma_B - 14) with level
     // weights 2^-7g, INT8)
     auto fold = [&](double scale) {
+      if constexpr (M32) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          double tsum = 0.0;
+#pragma unroll
+          for (int g = NG - 1; g >= 0; --g) {
+            tsum = fma((double)lvl32[g][v], 1.0 / (double)(1LL << (7 * g)), tsum);
+            lvl32[g][v] = 0;
+          }
+          acc32[v] = fma(tsum, scale, acc32[v]);
+        }
+        return;
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -620,7 +673,9 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     // register set would cost occupancy: the loads run one chunk ahead,
     // straight into LDS (measured: two chunks ahead through registers is
     // faster where it fits, 49.7 against 47.7 TFLOP/s for four BF16 limbs).
-    constexpr bool ONE_AHEAD = I8 ? S > 2 : S > 4;
+    // (The 32x32x32 blocks need fewer accumulator registers: up to four
+    // INT8 limbs they keep two register sets at two waves per SIMD.)
+    constexpr bool ONE_AHEAD = I8 ? (M32 ? S > 4 : S > 2) : S > 4;
     if constexpr (ONE_AHEAD) {
       if (lp < pend) {
         load_chunk_lds(0);
@@ -668,6 +723,21 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     const bool beta1 = (t_flags & TAMX_TASK_BETA1) != 0;
     double* __restrict__ c = p.C + t_coff;
     tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
+      if constexpr (M32) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          // 32x32 MFMA output: row 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32
+          const int row = wr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          const int col = wc + (lane & 31);
+          if (row < mrem && col < nrem) {
+            const int64_t idx = cidx(r0 + row, c0 + col);
+            double val = p.alpha * acc32[v];
+            if (beta1) val += c[idx];
+            c[idx] = val;
+          }
+        }
+        return;
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -698,8 +768,10 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
 
 template <int S, bool I8>
 int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm,
-                      hipStream_t s) {
+                      hipStream_t s, bool m32 = false) {
   auto kern = split_gemm_kernel<S, I8>;
+  if constexpr (I8)
+    if (m32) kern = split_gemm_kernel<S, I8, true>;
   int occ = 0;
   TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
   if (occ < 1) occ = 1;
@@ -1283,12 +1355,14 @@ int tamx_grouped_gemm_split_scatter(tamx_context* ctx, tamx_worklist* wl, double
     if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
       return launch_split_gemm_wide(TAMX_LIMB_I8, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
                                     prm, s);
+    // bk 64: the 32x32x32 blocks
+    const bool m32 = cfg && cfg->bk == SBK8;
     switch (a_plan->nslices) {
-      case 2: return launch_split_gemm<2, true>(ctx, wl, prm, s);
-      case 3: return launch_split_gemm<3, true>(ctx, wl, prm, s);
-      case 4: return launch_split_gemm<4, true>(ctx, wl, prm, s);
-      case 5: return launch_split_gemm<5, true>(ctx, wl, prm, s);
-      default: return launch_split_gemm<6, true>(ctx, wl, prm, s);
+      case 2: return launch_split_gemm<2, true>(ctx, wl, prm, s, m32);
+      case 3: return launch_split_gemm<3, true>(ctx, wl, prm, s, m32);
+      case 4: return launch_split_gemm<4, true>(ctx, wl, prm, s, m32);
+      case 5: return launch_split_gemm<5, true>(ctx, wl, prm, s, m32);
+      default: return launch_split_gemm<6, true>(ctx, wl, prm, s, m32);
     }
   }
   if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))

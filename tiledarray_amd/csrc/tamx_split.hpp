@@ -35,6 +35,7 @@ struct SplitTile {
 
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef int i16_t __attribute__((ext_vector_type(16)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 

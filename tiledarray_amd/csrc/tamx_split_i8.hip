@@ -7,17 +7,22 @@
 // results are identical bit for bit: every level sum is exact and the folds
 // happen at the same points in the same order.
 //
-// What differs is how the limbs reach the MFMAs.  A workgroup of two waves
-// computes a 64 x 64 macro tile, each wave 64 rows x 32 columns (4 x 2
-// blocks of 16 x 16), so every limb fragment a wave reads feeds four or
-// two blocks instead of two.  The A limbs (shared by both waves) go from
-// global memory straight into a three-stage LDS ring; the B limbs (each
-// wave its own columns) go straight into registers, three sets in a ring.
-// Chunk c + 2 is loaded while chunk c is multiplied, one barrier per chunk.
-// Per 64-k chunk a wave reads 20 KB of fragments from LDS and 10 KB from
-// global memory against 120 MFMAs (1920 cycles), half the LDS traffic per
-// MFMA of the four-wave kernel.  One wave per SIMD: the level and FP64
-// accumulators alone take 224 registers.
+// What differs is the MFMA and how the limbs reach it.  The products run
+// on the INT8 MFMA 32x32x32, which on this part does twice the work of the
+// 16x16x64 in about the same issue time (measured: 4.07 against 2.29 INT8
+// POP/s in a register-only loop).  A workgroup of two waves computes a
+// 64 x 64 macro tile, each wave 64 rows x 32 columns (two 32 x 32 blocks),
+// so every B fragment feeds two blocks.  The A limbs (shared by both waves)
+// go from global memory straight into a three-stage LDS ring; the B limbs
+// (each wave its own columns) go straight into registers, three sets in a
+// ring.  Chunk c + 2 is loaded while chunk c is multiplied, one barrier per
+// chunk.  One wave per SIMD: the level and FP64 accumulators alone take 224
+// registers.
+//
+// 32x32x32 INT8 operand layout (checked on the device): lane l holds row
+// (A) or column (B) l % 32 and the 16 k from 16 (l / 32) of the 32-k step;
+// result register v of lane l is row 8 (v / 4) + 4 (l / 32) + v % 4,
+// column l % 32.
 #include <type_traits>
 
 #include "tamx_internal.hpp"
@@ -26,6 +31,7 @@
 namespace {
 
 using namespace tamx::split;
+
 
 constexpr int kWaves = 2;
 constexpr int kT = 64 * kWaves;  // threads per workgroup
@@ -51,7 +57,6 @@ void split_gemm_i8_kernel(SplitParams p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = uni(tid >> 6);
   const int wc = wave * 32;                    // the wave's columns
-  const int lrow = lane & 15, lq = lane >> 4;  // MFMA: row lrow, k unit lq
   const auto* LA = reinterpret_cast<const i4_t*>(p.LA);
   const auto* LB = reinterpret_cast<const i4_t*>(p.LB);
 
@@ -102,24 +107,20 @@ void split_gemm_i8_kernel(SplitParams p) {
     const int tiles_n = (t_n + IBN - 1) / IBN;
     const int r0 = (local / tiles_n) * IBM, c0 = (local % tiles_n) * IBN;
     const int mrem = min(IBM, t_m - r0), nrem = min(IBN, t_n - c0);
-    // 16-row blocks inside the tile (both waves) and the wave's 16-column
-    // blocks inside it; a wave with none skips its MFMAs
-    const int ni = uni(min(4, (mrem + 15) / 16));
-    const int nj = uni(max(0, min(2, (nrem - wc + 15) / 16)));
-    const bool wave_active = nj > 0;
+    // 32-row blocks inside the tile (both waves); a wave whose 32 columns
+    // lie wholly past the tile's edge skips its MFMAs
+    const int ni = uni(min(2, (mrem + 31) / 32));
+    const bool wave_active = wc < nrem;
 
-    d4_t acc[4][2];
-    i4_t lvl[NG][4][2];
+    double acc[2][16];
+    i16_t lvl[NG][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int v = 0; v < 16; ++v) {
+        acc[i][v] = 0.0;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) lvl[g][i][j][v] = 0;
+        for (int g = 0; g < NG; ++g) lvl[g][i][v] = 0;
       }
 
     const int pend = t_pb + t_pc;
@@ -127,7 +128,8 @@ void split_gemm_i8_kernel(SplitParams p) {
     // load cursor: pair lp, chunk lk0, two chunks ahead of compute.  Per
     // pair: this thread's A units (16-byte units of the limb pool) of its
     // kALoads slots in plane 0 at k = 0, the plane strides, and its B
-    // fragment units (rows c0 + wc + 16 j + lrow, k unit lq).
+    // fragment unit (row c0 + wc + lane % 32, k unit lane / 32; the second
+    // 32-k step of a chunk is two units further).
     int lp = t_pb, lk0 = 0, lkp = 0;
     int64_t a_unit[kALoads], b_unit[2], a_plane = 0, b_plane = 0;
     auto set_pair = [&](int pi) {
@@ -148,7 +150,8 @@ void split_gemm_i8_kernel(SplitParams p) {
         a_unit[h] = a0 + (int64_t)(r0 + r) * kp16 + q;
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b_unit[j] = b0 + (int64_t)(c0 + wc + 16 * j + lrow) * kp16 + lq;
+      for (int h = 0; h < 2; ++h)
+        b_unit[h] = b0 + (int64_t)(c0 + wc + (lane & 31)) * kp16 + (lane >> 5) + 2 * h;
     };
     if (lp < pend) set_pair(lp);
     // Loads of the cursor's chunk: A into LDS stage `st`, B into `bs`.
@@ -165,7 +168,7 @@ void split_gemm_i8_kernel(SplitParams p) {
               (const __attribute__((address_space(1))) void*)(LA + a_unit[h] + s * a_plane + k16),
               (__attribute__((address_space(3))) void*)(&sA[st][s][kT * h + 64 * wave]), 16, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bs[s][j] = *tamx::global_ptr(LB + b_unit[j] + s * b_plane + k16);
+        for (int h = 0; h < 2; ++h) bs[s][h] = *tamx::global_ptr(LB + b_unit[h] + s * b_plane + k16);
       }
       if (lp < pend) {
         lk0 += CK;
@@ -177,42 +180,33 @@ void split_gemm_i8_kernel(SplitParams p) {
       }
     };
 
-    // The wave's 16-row blocks i < ni and 16-column blocks j < nj lie inside
-    // the tile (uniform per wave).  Block rows are multiplied one after the
-    // other, each behind its own uniform branch, so a ragged edge costs no
-    // MFMAs; the next block row's A fragments are read from LDS (always,
-    // rows past the edge hold whatever lies there) while the current one's
-    // MFMAs run.
+    // The wave's 32-row blocks i < ni lie inside the tile (uniform per
+    // wave).  Block rows and the chunk's two 32-k steps are multiplied one
+    // after the other, each block behind its own uniform branch, so a ragged
+    // edge costs no MFMAs; the next step's A fragments are read from LDS
+    // (always: rows past the edge hold whatever lies there) while the
+    // current step's MFMAs run.  bs[t][h]: limb t of the B fragment for
+    // k step h.
     auto compute_chunk = [&](int st, const i4_t (&bs)[S][2]) {
       if (!wave_active) return;
+      auto read_a = [&](i4_t (&fa)[S], int i, int h) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          fa[s] = sA[st][s][lds_unit(32 * i + (lane & 31), 2 * h + (lane >> 5))];
+      };
       i4_t fa[2][S];
+      read_a(fa[0], 0, 0);
 #pragma unroll
-      for (int s = 0; s < S; ++s) fa[0][s] = sA[st][s][lds_unit(lrow, lq)];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int cur = i & 1;
-        if (i < 3) {
-#pragma unroll
-          for (int s = 0; s < S; ++s) fa[cur ^ 1][s] = sA[st][s][lds_unit((i + 1) * 16 + lrow, lq)];
-        }
+      for (int step = 0; step < 4; ++step) {
+        const int i = step >> 1, h = step & 1, cur = step & 1;
+        if (step < 3) read_a(fa[cur ^ 1], (step + 1) >> 1, (step + 1) & 1);
         if (i < ni) {
-          if (nj == 2) {
 #pragma unroll
-            for (int s = 0; s < S; ++s)
+          for (int s = 0; s < S; ++s)
 #pragma unroll
-              for (int t = 0; t < S - s; ++t)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                  lvl[s + t][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                      fa[cur][s], bs[t][j], lvl[s + t][i][j], 0, 0, 0);
-          } else {
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-#pragma unroll
-              for (int t = 0; t < S - s; ++t)
-                lvl[s + t][i][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                    fa[cur][s], bs[t][0], lvl[s + t][i][0], 0, 0, 0);
-          }
+            for (int t = 0; t < S - s; ++t)
+              lvl[s + t][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][s], bs[t][h],
+                                                                     lvl[s + t][i], 0, 0, 0);
         }
       }
     };
@@ -221,20 +215,18 @@ void split_gemm_i8_kernel(SplitParams p) {
     // the edge hold zeros and are skipped
     auto fold = [&](double scale) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 2; ++i) {
         if (i < ni) {
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
+          for (int v = 0; v < 16; ++v) {
+            double tsum = 0.0;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              double tsum = 0.0;
-#pragma unroll
-              for (int g = NG - 1; g >= 0; --g) {
-                tsum = fma((double)lvl[g][i][j][v], 1.0 / (double)(1LL << (7 * g)), tsum);
-                lvl[g][i][j][v] = 0;
-              }
-              acc[i][j][v] = fma(tsum, scale, acc[i][j][v]);
+            for (int g = NG - 1; g >= 0; --g) {
+              tsum = fma((double)lvl[g][i][v], 1.0 / (double)(1LL << (7 * g)), tsum);
+              lvl[g][i][v] = 0;
             }
+            acc[i][v] = fma(tsum, scale, acc[i][v]);
+          }
         }
       }
     };
@@ -292,21 +284,19 @@ void split_gemm_i8_kernel(SplitParams p) {
     double* __restrict__ c = p.C + t_coff;
     tamx::with_result_index(p.c_scatter, p.task_tile, lo, t_flags, t_ldc, [&](auto cidx) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            // 16x16 MFMA output: row 4 * (lane / 16) + v, column lane % 16
-            const int row = i * 16 + lq * 4 + v;
-            const int col = wc + j * 16 + lrow;
-            if (row < mrem && col < nrem) {
-              const int64_t idx = cidx(r0 + row, c0 + col);
-              double val = p.alpha * acc[i][j][v];
-              if (beta1) val += c[idx];
-              c[idx] = val;
-            }
+        for (int v = 0; v < 16; ++v) {
+          // 32x32 MFMA output: row 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32
+          const int row = 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          const int col = wc + (lane & 31);
+          if (row < mrem && col < nrem) {
+            const int64_t idx = cidx(r0 + row, c0 + col);
+            double val = p.alpha * acc[i][v];
+            if (beta1) val += c[idx];
+            c[idx] = val;
           }
+        }
     });
   }
 
