@@ -76,6 +76,24 @@ struct GemmParams {
 };
 
 constexpr int kThreads = 256;
+
+// Buffer loads of plain operands: descriptor word 3 for raw 32-bit-format
+// buffers on this family, the offset that lies outside every range, and the
+// largest leading dimension whose per-element offsets stay below it.
+constexpr int kBufFlags = 0x00020000;
+constexpr uint32_t kOutside = 0x80000000u;
+constexpr int kBufLd = 1 << 20;
+
+template <typename T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t d, uint32_t off);
+template <>
+__device__ inline double buffer_load<double>(__amdgpu_buffer_rsrc_t d, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(d, off, 0, 0));
+}
+template <>
+__device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t d, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(d, off, 0, 0));
+}
 constexpr int kRegions = 8;   // XCDs of an MI355X; the queue holds kRegions + 1 counters
 constexpr int kPairWin = 64;  // pair descriptors staged in LDS at a time
 
@@ -103,7 +121,7 @@ __device__ inline int64_t uni64(int64_t x) {
 // pick only which index the staging lanes run along (A: NoTrans k, Trans m;
 // B: NoTrans n, Trans k).
 template <typename T, int BM, int BN, int BK, bool TA, bool TB, int GM>
-__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p) {
+__global__ __launch_bounds__(kThreads, (sizeof(T) == 8 && BM == 64 && BN == 64 && BK == 16) ? 4 : 1) void grouped_gemm_kernel(GemmParams<T> p) {
   constexpr bool GA = (GM & 1) != 0, GB = (GM & 2) != 0, G = GM != 0;
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -193,11 +211,9 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     // MFMAs, and inside a quadrant every 16 x 16 block that does (the
     // conditions are uniform across the wave)
     const bool wave_active = (wr < mrem) && (wc < nrem);
-    bool row_on[FM], col_on[FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i) row_on[i] = wr + i * 16 < mrem;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) col_on[j] = wc + j * 16 < nrem;
+    // 16-row (column) blocks of the quadrant inside the tile, as scalars
+    const int ni = uni(min(FM, max(0, (mrem - wr + 15) >> 4)));
+    const int nj = uni(min(FN, max(0, (nrem - wc + 15) >> 4)));
 
     acc_t acc[FM][FN];
 #pragma unroll
@@ -214,6 +230,15 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     int a_lda = 0, b_ldb = 0, pk = 0;
     int64_t a_off[A_LD], b_off[B_LD];
     bool a_ok[A_LD], b_ok[B_LD];
+    // Plain operands are read with buffer loads: per chunk one descriptor
+    // per operand based at the macro tile's first row (column) and the
+    // chunk's first k, and per element a 32-bit byte offset from it that is
+    // fixed for the pair.  An element outside the tile carries kOutside,
+    // beyond every descriptor's range, and loads zero; so do the k rows past
+    // the pair's end where k is the slow index (the range stops there).
+    // Pairs with a leading dimension above kBufLd take the pointer path.
+    uint32_t a_vo[A_LD], b_vo[B_LD];
+    bool buf_ok = false;
     // k-mode maps of the current pair's gathered tiles
     tamx::GatherMode ga_k[4], gb_k[4];
     int at = 0, bt = 0;
@@ -252,26 +277,42 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
           bt = uni(pp.b_tile);
         }
       }
+      if constexpr (!G) {
+        buf_ok = a_lda <= kBufLd && b_ldb <= kBufLd;
 #pragma unroll
-      for (int e = 0; e < A_LD; ++e) {
-        const int slow = a_slow + e * A_STEP;
-        if (TA) {  // (k = slow, m = a_fast)
-          a_ok[e] = a_fast < mrem;
-          a_off[e] = (int64_t)slow * a_lda + r0 + a_fast;
-        } else {  // (m = slow, k = a_fast)
-          a_ok[e] = slow < mrem;
-          a_off[e] = (int64_t)(r0 + slow) * a_lda + a_fast;
+        for (int e = 0; e < A_LD; ++e) {
+          const int slow = a_slow + e * A_STEP;
+          const bool in = TA ? a_fast < mrem : slow < mrem;
+          a_vo[e] = in ? (uint32_t)(slow * a_lda + a_fast) * (uint32_t)sizeof(T) : kOutside;
         }
-      }
 #pragma unroll
-      for (int e = 0; e < B_LD; ++e) {
-        const int slow = b_slow + e * B_STEP;
-        if (TB) {  // (n = slow, k = b_fast)
-          b_ok[e] = slow < nrem;
-          b_off[e] = (int64_t)(c0 + slow) * b_ldb + b_fast;
-        } else {  // (k = slow, n = b_fast)
-          b_ok[e] = b_fast < nrem;
-          b_off[e] = (int64_t)slow * b_ldb + c0 + b_fast;
+        for (int e = 0; e < B_LD; ++e) {
+          const int slow = b_slow + e * B_STEP;
+          const bool in = TB ? slow < nrem : b_fast < nrem;
+          b_vo[e] = in ? (uint32_t)(slow * b_ldb + b_fast) * (uint32_t)sizeof(T) : kOutside;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < A_LD; ++e) {
+          const int slow = a_slow + e * A_STEP;
+          if (TA) {  // (k = slow, m = a_fast)
+            a_ok[e] = a_fast < mrem;
+            a_off[e] = (int64_t)slow * a_lda + r0 + a_fast;
+          } else {  // (m = slow, k = a_fast)
+            a_ok[e] = slow < mrem;
+            a_off[e] = (int64_t)(r0 + slow) * a_lda + a_fast;
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < B_LD; ++e) {
+          const int slow = b_slow + e * B_STEP;
+          if (TB) {  // (n = slow, k = b_fast)
+            b_ok[e] = slow < nrem;
+            b_off[e] = (int64_t)(c0 + slow) * b_ldb + b_fast;
+          } else {  // (k = slow, n = b_fast)
+            b_ok[e] = b_fast < nrem;
+            b_off[e] = (int64_t)slow * b_ldb + c0 + b_fast;
+          }
         }
       }
       // gathered operands: row offsets of the thread's rows through the
@@ -316,8 +357,61 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
     // waits for the loads before they are stored to LDS.  Once the cursor
     // has passed the last pair the loads repeat its first chunk, unused:
     // every step issues the same loads, which keeps the wait counts exact.
-    auto load_chunk = [&](T (&ra)[A_LD], T (&rb)[B_LD]) {
+    // returns the number of 4-k MFMA steps the chunk holds (the rest of a
+    // pair's last chunk is zero padding, skipped)
+    auto load_chunk = [&](T (&ra)[A_LD], T (&rb)[B_LD]) -> int {
       const int kc = min(BK, pk - lk0);
+      if constexpr (!G) {
+        if (buf_ok) {
+          // chunk origins (elements) and ranges (bytes): k-slow operands
+          // end at the pair's last k row, k-fast ones rely on the marks
+          const int64_t a_org = uni64(a_base + (TA ? (int64_t)lk0 * a_lda + r0
+                                                    : (int64_t)r0 * a_lda + lk0));
+          const int64_t b_org = uni64(b_base + (TB ? (int64_t)c0 * b_ldb + lk0
+                                                    : (int64_t)lk0 * b_ldb + c0));
+          const int a_rng = TA ? uni(kc * a_lda * (int)sizeof(T)) : (int)(kOutside - 1);
+          const int b_rng = TB ? (int)(kOutside - 1) : uni(kc * b_ldb * (int)sizeof(T));
+          const auto ra_d = __builtin_amdgcn_make_buffer_rsrc((void*)(p.A + a_org), (short)0,
+                                                              a_rng, kBufFlags);
+          const auto rb_d = __builtin_amdgcn_make_buffer_rsrc((void*)(p.B + b_org), (short)0,
+                                                              b_rng, kBufFlags);
+          if (kc == BK || (TA && !TB)) {
+#pragma unroll
+            for (int e = 0; e < A_LD; ++e) ra[e] = buffer_load<T>(ra_d, a_vo[e]);
+#pragma unroll
+            for (int e = 0; e < B_LD; ++e) rb[e] = buffer_load<T>(rb_d, b_vo[e]);
+          } else {
+            // a pair's last chunk: k-fast operands mask the k past its end
+            const bool a_kok = TA || a_fast < kc, b_kok = !TB || b_fast < kc;
+#pragma unroll
+            for (int e = 0; e < A_LD; ++e) ra[e] = buffer_load<T>(ra_d, a_kok ? a_vo[e] : kOutside);
+#pragma unroll
+            for (int e = 0; e < B_LD; ++e) rb[e] = buffer_load<T>(rb_d, b_kok ? b_vo[e] : kOutside);
+          }
+        } else {
+          // 64-bit addresses, recomputed per chunk (no state kept per pair)
+          const T* __restrict__ a = p.A + a_base;
+          const T* __restrict__ b = p.B + b_base;
+#pragma unroll
+          for (int e = 0; e < A_LD; ++e) {
+            const int slow = a_slow + e * A_STEP;
+            const bool ok = TA ? (a_fast < mrem && slow < kc) : (slow < mrem && a_fast < kc);
+            const int64_t off = TA ? (int64_t)(lk0 + slow) * a_lda + r0 + a_fast
+                                   : (int64_t)(r0 + slow) * a_lda + lk0 + a_fast;
+            ra[e] = *(ok ? tamx::global_ptr(a + off) : tamx::global_ptr(zero_of<T>()));
+          }
+#pragma unroll
+          for (int e = 0; e < B_LD; ++e) {
+            const int slow = b_slow + e * B_STEP;
+            const bool ok = TB ? (slow < nrem && b_fast < kc) : (b_fast < nrem && slow < kc);
+            const int64_t off = TB ? (int64_t)(c0 + slow) * b_ldb + lk0 + b_fast
+                                   : (int64_t)(lk0 + slow) * b_ldb + c0 + b_fast;
+            rb[e] = *(ok ? tamx::global_ptr(b + off) : tamx::global_ptr(zero_of<T>()));
+          }
+        }
+        if (lp < pend) advance();
+        return uni((kc + 3) >> 2);
+      }
       const T* __restrict__ a = p.A + a_base;
       const T* __restrict__ b = p.B + b_base;
       const int64_t a_k = TA ? (int64_t)lk0 * a_lda : lk0;
@@ -344,61 +438,90 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(GemmParams<T> p)
         rb[e] = *(ok ? tamx::global_ptr(b + b_off[e] + ko) : tamx::global_ptr(zero_of<T>()));
       }
       if (lp < pend) advance();
+      return uni((kc + 3) >> 2);
     };
-    auto store_chunk = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD]) {
+    // stores part `part` of NP parts of a staged chunk (all of it: part 0 of 1)
+    auto store_part = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD], int part, int NP) {
 #pragma unroll
       for (int e = 0; e < A_LD; ++e) {
+        if (e * NP / A_LD != part) continue;
         if (TA) sA[buf][a_slow + e * A_STEP][a_fast] = ra[e];
         else sA[buf][a_fast][a_slow + e * A_STEP] = ra[e];
       }
 #pragma unroll
       for (int e = 0; e < B_LD; ++e) {
+        if (e * NP / B_LD != part) continue;
         if (TB) sB[buf][b_fast][b_slow + e * B_STEP] = rb[e];
         else sB[buf][b_slow + e * B_STEP][b_fast] = rb[e];
       }
     };
-    auto compute_chunk = [&](int buf) {
-      if (!wave_active) return;
+    auto store_chunk = [&](int buf, const T (&ra)[A_LD], const T (&rb)[B_LD]) {
+      store_part(buf, ra, rb, 0, 1);
+    };
+    // nks 4-k steps of LDS buffer `buf`; the fragments of step s + 1 are
+    // read while the MFMAs of step s run (the reads of blocks past a ragged
+    // edge are harmless: they stay inside the buffer)
+    // `stage(st)` runs after the MFMAs of step st (every step, also those
+    // skipped): the caller stages the next chunk into the other buffer there,
+    // so the wave's LDS stores overlap its own MFMAs.
+    auto compute_chunk = [&](int buf, int nks, auto&& stage) {
+      if (!wave_active) {
 #pragma unroll
-      for (int ks = 0; ks < BK; ks += 4) {
-        T af[FM], bf[FN];
+        for (int st = 0; st < BK / 4; ++st) stage(st);
+        return;
+      }
+      T af[2][FM], bf[2][FN];
+      auto frag = [&](int ks, T (&a)[FM], T (&b)[FN]) {
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
+        for (int i = 0; i < FM; ++i) a[i] = sA[buf][ks + lk][wr + i * 16 + lrow];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
+        for (int j = 0; j < FN; ++j) b[j] = sB[buf][ks + lk][wc + j * 16 + lrow];
+      };
+      frag(0, af[0], bf[0]);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+      for (int st = 0; st < BK / 4; ++st) {
+        if (st < nks) {
+          if (st + 1 < BK / 4) frag(4 * (st + 1), af[(st + 1) & 1], bf[(st + 1) & 1]);
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
-            if (row_on[i] && col_on[j]) acc[i][j] = M::op(af[i], bf[j], acc[i][j]);
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              if (i < ni && j < nj) acc[i][j] = M::op(af[st & 1][i], bf[st & 1][j], acc[i][j]);
+        }
+        stage(st);
       }
     };
 
     // Pipeline: registers R0/R1 hold chunks c+1 and c+2 in flight while
     // chunk c is multiplied out of LDS buffer c % 2.  One barrier per chunk.
+    // (n0, n1: 4-k steps of the chunks in R0 / R1)
     T ra0[A_LD], rb0[B_LD], ra1[A_LD], rb1[B_LD];
     if (lp < pend) {
-      load_chunk(ra0, rb0);                    // chunk 0
+      int n0 = load_chunk(ra0, rb0);           // chunk 0
       const bool have1 = lp < pend;
-      load_chunk(ra1, rb1);                    // chunk 1 (if have1)
+      int n1 = load_chunk(ra1, rb1);           // chunk 1 (if have1)
       store_chunk(0, ra0, rb0);
+      int nb = n0;  // steps of the chunk in the buffer being computed
       __syncthreads();
       bool more = have1;  // chunk c + 1 exists
+      // (the stores of chunk c + 1 run between the MFMA steps of chunk c;
+      // past the last chunk they store the unused repeat, harmlessly)
+      constexpr int NS = BK / 4;
       for (;;) {
         // even step: compute buffer 0, chunk c+1 is in R1
         bool next2 = lp < pend;
-        load_chunk(ra0, rb0);                  // chunk c + 2 -> R0 (if next2)
-        compute_chunk(0);
+        n0 = load_chunk(ra0, rb0);             // chunk c + 2 -> R0 (if next2)
+        compute_chunk(0, nb, [&](int st) { store_part(1, ra1, rb1, st, NS); });
         if (!more) break;
-        store_chunk(1, ra1, rb1);
+        nb = n1;
         __syncthreads();
         more = next2;
         // odd step: compute buffer 1, chunk c+1 is in R0
         next2 = lp < pend;
-        load_chunk(ra1, rb1);                  // chunk c + 2 -> R1 (if next2)
-        compute_chunk(1);
+        n1 = load_chunk(ra1, rb1);             // chunk c + 2 -> R1 (if next2)
+        compute_chunk(1, nb, [&](int st) { store_part(0, ra0, rb0, st, NS); });
         if (!more) break;
-        store_chunk(0, ra0, rb0);
+        nb = n0;
         __syncthreads();
         more = next2;
       }

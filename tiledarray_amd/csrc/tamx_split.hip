@@ -378,9 +378,21 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
   using lvl_t = std::conditional_t<I8, i4_t, f4_t>;
   // LDS holds only the limb chunks (S * 16 KB); the grabbed item is passed
   // through the first word of sA while the buffers are idle between items
-  __shared__ bf8_t sA[2][S][SBM * 4];
-  __shared__ bf8_t sB[2][S][SBN * 4];
-  int* s_item = reinterpret_cast<int*>(&sA[0][0][0]);
+  // (one variable per buffer: the compiler then knows that loads landing in
+  // one buffer do not alias reads of the other, and waits for them only
+  // where the buffer they fill is read)
+  __shared__ bf8_t sA0[S][SBM * 4], sA1[S][SBM * 4];
+  __shared__ bf8_t sB0[S][SBN * 4], sB1[S][SBN * 4];
+  int* s_item = reinterpret_cast<int*>(&sA0[0][0]);
+  using zero = std::integral_constant<int, 0>;
+  using one = std::integral_constant<int, 1>;
+  using two = std::integral_constant<int, 2>;
+  auto bufA = [&](auto B) -> bf8_t(&)[S][SBM * 4] {
+    if constexpr (decltype(B)::value == 0) return sA0; else return sA1;
+  };
+  auto bufB = [&](auto B) -> bf8_t(&)[S][SBN * 4] {
+    if constexpr (decltype(B)::value == 0) return sB0; else return sB1;
+  };
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = uni(tid >> 6);
@@ -510,16 +522,16 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     // q = (tid % 4) ^ ((st_r / 2) % 4) of that row.  The caller waits for
     // the loads (vmcnt) before the barrier that precedes reading them.
     const int st_qs = st_q ^ ((st_r >> 1) & 3);
-    auto load_chunk_lds = [&](int buf) {
+    auto load_chunk_lds = [&](auto buf) {
       const int64_t q = (lk0 >> USH) + st_qs;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         __builtin_amdgcn_global_load_lds(
             (const __attribute__((address_space(1))) void*)(p.LA + la_row + s * la_plane + q),
-            (__attribute__((address_space(3))) void*)(&sA[buf][s][wave * 64]), 16, 0, 0);
+            (__attribute__((address_space(3))) void*)(&bufA(buf)[s][wave * 64]), 16, 0, 0);
         __builtin_amdgcn_global_load_lds(
             (const __attribute__((address_space(1))) void*)(p.LB + lb_row + s * lb_plane + q),
-            (__attribute__((address_space(3))) void*)(&sB[buf][s][wave * 64]), 16, 0, 0);
+            (__attribute__((address_space(3))) void*)(&bufB(buf)[s][wave * 64]), 16, 0, 0);
       }
       if (lp < pend) {
         lk0 += CK;
@@ -530,28 +542,28 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
         }
       }
     };
-    auto store_chunk = [&](int buf, const bf8_t (&ra)[S], const bf8_t (&rb)[S]) {
+    auto store_chunk = [&](auto buf, const bf8_t (&ra)[S], const bf8_t (&rb)[S]) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        sA[buf][s][lds_unit(st_r, st_q)] = ra[s];
-        sB[buf][s][lds_unit(st_r, st_q)] = rb[s];
+        bufA(buf)[s][lds_unit(st_r, st_q)] = ra[s];
+        bufB(buf)[s][lds_unit(st_r, st_q)] = rb[s];
       }
     };
     // NI x NJ of the wave's 2 x 2 blocks hold rows and columns of the tile
     // (a ragged edge leaves the rest empty; the split is uniform per wave)
     const int ni = wr + 16 < mrem ? 2 : 1, nj = wc + 16 < nrem ? 2 : 1;
-    auto compute_blocks = [&](int buf, auto NI_, auto NJ_) {
+    auto compute_blocks = [&](auto buf, auto NI_, auto NJ_) {
       constexpr int NI = decltype(NI_)::value, NJ = decltype(NJ_)::value;
       bf8_t bf[NJ][S];
 #pragma unroll
       for (int j = 0; j < NJ; ++j)
 #pragma unroll
-        for (int t = 0; t < S; ++t) bf[j][t] = sB[buf][t][lds_unit(wc + j * 16 + lrow, lq)];
+        for (int t = 0; t < S; ++t) bf[j][t] = bufB(buf)[t][lds_unit(wc + j * 16 + lrow, lq)];
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         bf8_t af[NI];
 #pragma unroll
-        for (int i = 0; i < NI; ++i) af[i] = sA[buf][s][lds_unit(wr + i * 16 + lrow, lq)];
+        for (int i = 0; i < NI; ++i) af[i] = bufA(buf)[s][lds_unit(wr + i * 16 + lrow, lq)];
 #pragma unroll
         for (int t = 0; t < S - s; ++t) {
           const int g = I8 ? s + t : (s + t < 2 ? s + t : 2);
@@ -570,9 +582,7 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
         }
       }
     };
-    using one = std::integral_constant<int, 1>;
-    using two = std::integral_constant<int, 2>;
-    auto compute_chunk = [&](int buf) {
+    auto compute_chunk = [&](auto buf) {
       if (!wave_active) return;
       if constexpr (M32) {
         // the quadrant as one 32 x 32 block; two 32-k steps per chunk,
@@ -584,11 +594,11 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
 #pragma unroll
           for (int t = 0; t < S; ++t)
             bf[t] = __builtin_bit_cast(
-                i4_t, sB[buf][t][lds_unit(wc + (lane & 31), 2 * h + (lane >> 5))]);
+                i4_t, bufB(buf)[t][lds_unit(wc + (lane & 31), 2 * h + (lane >> 5))]);
 #pragma unroll
           for (int s = 0; s < S; ++s) {
             const i4_t af = __builtin_bit_cast(
-                i4_t, sA[buf][s][lds_unit(wr + (lane & 31), 2 * h + (lane >> 5))]);
+                i4_t, bufA(buf)[s][lds_unit(wr + (lane & 31), 2 * h + (lane >> 5))]);
 #pragma unroll
             for (int t = 0; t < S - s; ++t)
               lvl32[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[t], lvl32[s + t], 0, 0, 0);
@@ -678,43 +688,48 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     constexpr bool ONE_AHEAD = I8 ? (M32 ? S > 4 : S > 2) : S > 4;
     if constexpr (ONE_AHEAD) {
       if (lp < pend) {
-        load_chunk_lds(0);
+        load_chunk_lds(zero{});
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        int buf = 0;
         for (;;) {
-          const bool has_next = lp < pend;
-          if (has_next) load_chunk_lds(buf ^ 1);  // chunk c+1 -> the idle buffer
-          compute_chunk(buf);
+          bool has_next = lp < pend;
+          if (has_next) load_chunk_lds(one{});  // chunk c+1 -> the idle buffer
+          compute_chunk(zero{});
           after_compute();
           if (!has_next) break;
           __builtin_amdgcn_s_waitcnt(0);
           __syncthreads();
-          buf ^= 1;
+          has_next = lp < pend;
+          if (has_next) load_chunk_lds(zero{});
+          compute_chunk(one{});
+          after_compute();
+          if (!has_next) break;
+          __builtin_amdgcn_s_waitcnt(0);
+          __syncthreads();
         }
       }
     } else if (lp < pend) {
       load_chunk(ra0, rb0);                // chunk 0
       const bool have1 = lp < pend;
       load_chunk(ra1, rb1);                // chunk 1 (if have1)
-      store_chunk(0, ra0, rb0);
+      store_chunk(zero{}, ra0, rb0);
       __syncthreads();
       bool more = have1;  // chunk c + 1 exists
       for (;;) {
         bool next2 = lp < pend;
         load_chunk(ra0, rb0);              // chunk c + 2 -> R0 (if next2)
-        compute_chunk(0);
+        compute_chunk(zero{});
         after_compute();
         if (!more) break;
-        store_chunk(1, ra1, rb1);
+        store_chunk(one{}, ra1, rb1);
         __syncthreads();
         more = next2;
         next2 = lp < pend;
         load_chunk(ra1, rb1);              // chunk c + 2 -> R1 (if next2)
-        compute_chunk(1);
+        compute_chunk(one{});
         after_compute();
         if (!more) break;
-        store_chunk(0, ra0, rb0);
+        store_chunk(zero{}, ra0, rb0);
         __syncthreads();
         more = next2;
       }

@@ -42,6 +42,9 @@ constexpr int kALoads = kAUnits / kT;     // direct loads per thread per plane
 static_assert(kAUnits % kT == 0, "A chunk split over the threads");
 
 // vmcnt wait (the other counters left alone), gfx9 encoding
+template <int X>
+using st_t = std::integral_constant<int, X>;
+
 constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 
 template <int S>
@@ -51,8 +54,18 @@ void split_gemm_i8_kernel(SplitParams p) {
   constexpr int CK = SBK8;  // 64 k per chunk
   // vector memory operations per thread per chunk: A into LDS, B into registers
   constexpr int kOps = S * kALoads + S * 2;
-  __shared__ i4_t sA[kStages][S][kAUnits];
+  // one variable per stage: the compiler then knows that loads landing in
+  // one stage do not alias reads of another, and waits for them only where
+  // that stage is read
+  __shared__ i4_t sA0[S][kAUnits], sA1[S][kAUnits], sA2[S][kAUnits];
   __shared__ int s_item;
+  static_assert(kStages == 3, "three stage variables");
+  auto stage = [&](auto X) -> i4_t(&)[S][kAUnits] {
+    constexpr int x = decltype(X)::value;
+    if constexpr (x == 0) return sA0;
+    else if constexpr (x == 1) return sA1;
+    else return sA2;
+  };
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = uni(tid >> 6);
@@ -158,7 +171,7 @@ void split_gemm_i8_kernel(SplitParams p) {
     // Once the cursor has passed the last pair the loads repeat a chunk of
     // the last pair, unused, so every step issues the same number of loads
     // and the wait counts stay exact.
-    auto load_chunk = [&](int st, i4_t (&bs)[S][2]) {
+    auto load_chunk = [&](auto st, i4_t (&bs)[S][2]) {
       const int64_t k16 = lk0 >> 4;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
@@ -166,7 +179,7 @@ void split_gemm_i8_kernel(SplitParams p) {
         for (int h = 0; h < kALoads; ++h)
           __builtin_amdgcn_global_load_lds(
               (const __attribute__((address_space(1))) void*)(LA + a_unit[h] + s * a_plane + k16),
-              (__attribute__((address_space(3))) void*)(&sA[st][s][kT * h + 64 * wave]), 16, 0, 0);
+              (__attribute__((address_space(3))) void*)(&stage(st)[s][kT * h + 64 * wave]), 16, 0, 0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) bs[s][h] = *tamx::global_ptr(LB + b_unit[h] + s * b_plane + k16);
       }
@@ -187,12 +200,12 @@ void split_gemm_i8_kernel(SplitParams p) {
     // (always: rows past the edge hold whatever lies there) while the
     // current step's MFMAs run.  bs[t][h]: limb t of the B fragment for
     // k step h.
-    auto compute_chunk = [&](int st, const i4_t (&bs)[S][2]) {
+    auto compute_chunk = [&](auto st, const i4_t (&bs)[S][2]) {
       if (!wave_active) return;
       auto read_a = [&](i4_t (&fa)[S], int i, int h) {
 #pragma unroll
         for (int s = 0; s < S; ++s)
-          fa[s] = sA[st][s][lds_unit(32 * i + (lane & 31), 2 * h + (lane >> 5))];
+          fa[s] = stage(st)[s][lds_unit(32 * i + (lane & 31), 2 * h + (lane >> 5))];
       };
       i4_t fa[2][S];
       read_a(fa[0], 0, 0);
@@ -259,22 +272,22 @@ void split_gemm_i8_kernel(SplitParams p) {
 
     i4_t b0[S][2], b1[S][2], b2[S][2];
     if (lp < pend) {
-      load_chunk(0, b0);  // chunk 0
-      load_chunk(1, b1);  // chunk 1 (or a repeat)
+      load_chunk(st_t<0>{}, b0);  // chunk 0
+      load_chunk(st_t<1>{}, b1);  // chunk 1 (or a repeat)
       // one step: chunk c in stage X with B in bx; chunk c + 1 in flight;
       // chunk c + 2 goes to stage (X + 2) % 3, which held chunk c - 1
-      auto step = [&](int X, const i4_t (&bx)[S][2], i4_t (&bn)[S][2]) {
+      auto step = [&](auto X, const i4_t (&bx)[S][2], i4_t (&bn)[S][2]) {
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(kOps));
         __syncthreads();
-        load_chunk(X == 0 ? 2 : X - 1, bn);
+        load_chunk(st_t<(decltype(X)::value + 2) % 3>{}, bn);
         compute_chunk(X, bx);
         after_compute();
         return cp < pend;
       };
       for (;;) {
-        if (!step(0, b0, b2)) break;
-        if (!step(1, b1, b0)) break;
-        if (!step(2, b2, b1)) break;
+        if (!step(st_t<0>{}, b0, b2)) break;
+        if (!step(st_t<1>{}, b1, b0)) break;
+        if (!step(st_t<2>{}, b2, b1)) break;
       }
       // the repeats still in flight land in the ring before it is reused
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));

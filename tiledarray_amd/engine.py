@@ -450,7 +450,9 @@ class Contraction:
         self.c_off = torch.empty(self.Mt * self.Nt, dtype=torch.int64, device=dev)
         self.c_capacity = int(mext.sum() * next_.sum())
         self.dtype = A.pool.dtype
-        self.C = torch.empty(self.c_capacity, dtype=self.dtype, device=dev)
+        # (zeroed once: the part past the result tiles then holds no stale
+        # bits, so whole-buffer comparisons are well defined)
+        self.C = torch.zeros(self.c_capacity, dtype=self.dtype, device=dev)
         self.store_trans = bool(plan.result_perm_matrix_transpose)
         # a general result permutation (contract_reduce.h:370-378) is fused
         # into the GEMM's store: every result tile is written straight into
@@ -884,7 +886,7 @@ class BatchedContraction:
         self.d_next = torch.as_tensor(nd.astype(np.int32).reshape(-1), device=dev)
         self.d_kext = torch.as_tensor(kd.astype(np.int32).reshape(-1), device=dev)
         self.c_capacity = int(sum(int(md[b].sum()) * int(nd[b].sum()) for b in range(nbatch)))
-        self.C = torch.empty(max(1, self.c_capacity), dtype=self.dtype, device=dev)
+        self.C = torch.zeros(max(1, self.c_capacity), dtype=self.dtype, device=dev)
         self.c_off = torch.empty(nbatch * Mt * Nt, dtype=torch.int64, device=dev)
         grid = nbatch * Mt * Nt
         self.wl = WorkList(ctx, grid, grid * Kt, grid)
