@@ -455,8 +455,12 @@ int tamx_split_plan_create_general(tamx_context* ctx, int nslices, int64_t ntile
  * |l_s| <= 64 after it and |r| <= 2^(sigma - 7*nslices - 1) (sigma as above,
  * one more where the tile max would round to 128), multiplied on the INT8
  * MFMA with one exact INT32 accumulator per level s + t (an integer-slice
- * scheme in the manner of Ozaki et al.); limbs are laid out like the BF16
- * ones with K padded to 64 and one byte per limb. */
+ * scheme in the manner of Ozaki et al.); one byte per limb, the logical
+ * tile as m x K (n x K) with rows padded to 16 and K to 64, each plane
+ * stored in blocks of 32 K: block b holds K 32b .. 32b + 31 of every row,
+ * rows one after the other (limb (r, k) at ((k / 32) * m_pad + r) * 32 +
+ * k % 32), so the chunk a GEMM loads for a macro tile is one contiguous run
+ * per plane and block. */
 #define TAMX_LIMB_BF16 0
 #define TAMX_LIMB_I8 1
 int tamx_split_plan_create_typed(tamx_context* ctx, int limb_type, int nslices,
@@ -503,8 +507,10 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
 /* The same with a macro tile: bm x bn of 64 x 64 (the default, any limb
  * count), 128 x 64 or 64 x 128 (2 to 4 BF16 or 3 to 4 INT8 limbs; one wave
  * per SIMD, a quarter fewer limb bytes loaded per result).  For INT8 limbs
- * at 64 x 64, bk picks the kernel: 64, each wave's 32 x 32 quadrant on the
- * INT8 MFMA 32x32x32; 128 (4 or 5 limbs), two-wave workgroups with A limbs
+ * at 64 x 64, bk picks the kernel: 32, each wave's 32 x 32 quadrant on the
+ * INT8 MFMA 32x32x32 with 32-k chunks through a four-stage LDS ring (three
+ * chunks in flight; chunks of k padding skipped); 64, the same blocks with
+ * two 64-k buffers; 128 (4 or 5 limbs), two-wave workgroups with A limbs
  * through an LDS ring and B limbs straight into registers; otherwise the
  * 16x16x64 MFMA.  All give the same result bit for bit.  wgs_per_cu is
  * ignored. */

@@ -125,14 +125,18 @@ def test_split_wide_tiles_match_default(ctx, tile, precision):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
-@pytest.mark.parametrize("bk", [64, 128])
-@pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
+@pytest.mark.parametrize("bk", [32, 64, 128])
+@pytest.mark.parametrize("precision", ["i8x3", "i8x4", "i8x5"])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
 def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b, bk):
-    # the INT8 kernels on the 32x32x32 MFMA (64 x 64 macro tiles; bk 64:
-    # four waves, bk 128: two waves) on tiles ragged in both directions,
-    # narrower and wider than a macro tile
+    # the INT8 kernels on the 32x32x32 MFMA (64 x 64 macro tiles; bk 32: four
+    # waves with a four-stage ring of 32-k chunks, bk 64: four waves with two
+    # 64-k buffers, bk 128: two waves) on tiles ragged in both directions,
+    # narrower and wider than a macro tile, and k extents that are not
+    # multiples of a chunk (17, 47, 86, 1, 149)
     nslices = int(precision.split("x")[-1])
+    if bk == 128 and nslices not in (4, 5):
+        pytest.skip("the two-wave kernel takes 4 or 5 limbs")
     rng = np.random.default_rng(1300 + nslices * 4 + op_a * 2 + op_b)
     trm = TiledRange1([0, 40, 170, 300, 301])
     trk = TiledRange1([0, 17, 64, 150, 151, 300])
@@ -149,12 +153,14 @@ def test_split_i8_two_wave_integer_exact(ctx, precision, op_a, op_b, bk):
     np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
 
 
-@pytest.mark.parametrize("bk", [64, 128])
-@pytest.mark.parametrize("precision", ["i8x4", "i8x5"])
+@pytest.mark.parametrize("bk", [32, 64, 128])
+@pytest.mark.parametrize("precision", ["i8x2", "i8x4", "i8x5", "i8x6"])
 def test_split_i8_two_wave_matches_default(ctx, precision, bk):
     # same limbs, same pair order, same folds: the 32x32x32 kernels' results
     # equal the 16x16x64 kernel's bit for bit on random block-sparse data,
     # and on a second run over a smaller problem after it
+    if bk == 128 and precision not in ("i8x4", "i8x5"):
+        pytest.skip("the two-wave kernel takes 4 or 5 limbs")
     for n, seed in ((1500, 31), (700, 32)):
         A, B, _ = W.blocksparse_gemm(n=n, wlo=24, whi=150, seed=seed)
         E.make_sparse(ctx, A, 1e-8)
@@ -380,7 +386,9 @@ def _i8_split_tile(ctx, a, nslices, is_right, op):
     m, k = logical.shape
     mp, kp = -(-m // 16) * 16, -(-k // 64) * 64
     L = limbs.cpu().numpy().view(np.int8).astype(np.float64)[:nslices * mp * kp]
-    return L.reshape(nslices, mp, kp), int(sig[0]), logical
+    # each plane in blocks of 32 k: block b holds rows 0..mp-1, 32 limbs each
+    L = L.reshape(nslices, kp // 32, mp, 32).transpose(0, 2, 1, 3).reshape(nslices, mp, kp)
+    return L, int(sig[0]), logical
 
 
 def _i8_limbs_by_definition(logical, sigma, nslices):

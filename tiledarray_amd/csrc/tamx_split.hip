@@ -339,9 +339,11 @@ __global__ __launch_bounds__(256) void split_limbs_i8_kernel(
     for (int s = 0; s < S; ++s)
       out[s] = (uint64_t)low_bytes4(w[s][0], w[s][1], w[s][2], w[s][3]) |
                (uint64_t)low_bytes4(w[s][4], w[s][5], w[s][6], w[s][7]) << 32;
-    // in units of 8 limbs
+    // in units of 8 limbs, in 32-k blocks (i8_unit)
     const int64_t plane = ((int64_t)t.rows_pad * t.k_pad) >> 3;
-    const int64_t o = (t.limb_off + (int64_t)(r0 + r) * t.k_pad + k0 + 8 * q) >> 3;
+    const int kk = k0 + 8 * q;
+    const int64_t o = (t.limb_off + ((int64_t)(kk / kI8Block) * t.rows_pad + r0 + r) * kI8Block +
+                       kk % kI8Block) >> 3;
 #pragma unroll
     for (int s = 0; s < S; ++s) limbs[o + s * plane] = out[s];
   });
@@ -482,30 +484,47 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
 
     // load cursor: pair lp, chunk lk0; runs two chunks ahead of compute
     int lp = t_pb, lk0 = 0, lkp = 0;
-    int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0;
+    int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0, la_blk = 0, lb_blk = 0;
     auto set_pair = [&](int pi) {
       const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
       const int at = uni(pp.a_tile), bt = uni(pp.b_tile);
       const auto* A = tamx::scalar_ptr(p.ta) + at;
       const auto* B = tamx::scalar_ptr(p.tb) + bt;
       lkp = uni(A->k_pad);
-      // limb offsets of this thread's staging row in plane 0, in 16-byte units
+      // limb offsets of this thread's staging row in plane 0, in 16-byte
+      // units (INT8: at k 0 of the 32-k block layout, and the units per block)
       la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> USH;
       lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> USH;
-      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
-      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
+      if constexpr (I8) {
+        la_blk = 2 * uni(A->rows_pad);
+        lb_blk = 2 * uni(B->rows_pad);
+        la_row = i8_unit(uni64(A->limb_off) >> 4, 0, r0 + st_r, 0);
+        lb_row = i8_unit(uni64(B->limb_off) >> 4, 0, c0 + st_r, 0);
+      } else {
+        la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
+        lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
+      }
     };
     if (lp < pend) set_pair(lp);
+    // unit q of the chunk at lk0 in this thread's staging row of plane 0
+    auto a_unit = [&](int q) -> int64_t {
+      if constexpr (I8) return la_row + ((lk0 / kI8Block) + (q >> 1)) * la_blk + (q & 1);
+      else return la_row + (lk0 >> USH) + q;
+    };
+    auto b_unit = [&](int q) -> int64_t {
+      if constexpr (I8) return lb_row + ((lk0 / kI8Block) + (q >> 1)) * lb_blk + (q & 1);
+      else return lb_row + (lk0 >> USH) + q;
+    };
     // Once the cursor has passed the last pair the loads repeat its first
     // chunk, unused: every step issues the same loads, which keeps the wait
     // counts before the LDS stores exact (a conditional load would make the
     // compiler wait for all loads in flight).
     auto load_chunk = [&](bf8_t (&ra)[S], bf8_t (&rb)[S]) {
-      const int64_t q = (lk0 >> USH) + st_q;
+      const int64_t ua = a_unit(st_q), ub = b_unit(st_q);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        ra[s] = p.LA[la_row + s * la_plane + q];
-        rb[s] = p.LB[lb_row + s * lb_plane + q];
+        ra[s] = p.LA[ua + s * la_plane];
+        rb[s] = p.LB[ub + s * lb_plane];
       }
       if (lp < pend) {
         lk0 += CK;
@@ -523,14 +542,14 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     // the loads (vmcnt) before the barrier that precedes reading them.
     const int st_qs = st_q ^ ((st_r >> 1) & 3);
     auto load_chunk_lds = [&](auto buf) {
-      const int64_t q = (lk0 >> USH) + st_qs;
+      const int64_t ua = a_unit(st_qs), ub = b_unit(st_qs);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(p.LA + la_row + s * la_plane + q),
+            (const __attribute__((address_space(1))) void*)(p.LA + ua + s * la_plane),
             (__attribute__((address_space(3))) void*)(&bufA(buf)[s][wave * 64]), 16, 0, 0);
         __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(p.LB + lb_row + s * lb_plane + q),
+            (const __attribute__((address_space(1))) void*)(p.LB + ub + s * lb_plane),
             (__attribute__((address_space(3))) void*)(&bufB(buf)[s][wave * 64]), 16, 0, 0);
       }
       if (lp < pend) {
@@ -1367,6 +1386,8 @@ int tamx_grouped_gemm_split_scatter(tamx_context* ctx, tamx_worklist* wl, double
   if (a_plan->limb_type == TAMX_LIMB_I8) {
     if (cfg && cfg->bm == SBM && cfg->bn == SBN && cfg->bk == 2 * SBK8)
       return launch_split_gemm_i8_2w(a_plan->nslices, ctx, wl, prm, s);
+    if (cfg && cfg->bm == SBM && cfg->bn == SBN && cfg->bk == SBK8 / 2)
+      return launch_split_gemm_i8_ring(a_plan->nslices, ctx, wl, prm, s);
     if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
       return launch_split_gemm_wide(TAMX_LIMB_I8, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
                                     prm, s);

@@ -69,6 +69,16 @@ constexpr int kFoldChunks = 4;  // fold FP32 levels into FP64 every 128 k
 // their range
 constexpr int SBK8 = 64;
 constexpr int kFoldChunks8 = 64;
+// INT8 limb planes are stored in blocks of 32 k: within a block the 32 limbs
+// of a row are contiguous and the block's rows_pad rows follow each other, so
+// the chunk of a macro tile's rows that a GEMM loads is one contiguous run
+// per plane and block (whole cache lines, rather than 32 or 64 bytes per row).
+constexpr int kI8Block = 32;
+// the 16-limb unit u (k 16u .. 16u + 15) of row r in a plane starting at
+// unit `plane0`
+__host__ __device__ inline int64_t i8_unit(int64_t plane0, int rows_pad, int r, int u) {
+  return plane0 + ((int64_t)(u >> 1) * rows_pad + r) * 2 + (u & 1);
+}
 // rows of slack past the last tile of a limb pool: a macro tile may read up
 // to this many rows beyond a tile's padded rows (those rows only reach
 // results that are not stored)
@@ -97,6 +107,11 @@ int launch_split_gemm_wide(int limb_type, int nslices, int bm, int bn, tamx_cont
 // registers (tamx_split_i8.hip); plans its items and launches.
 int launch_split_gemm_i8_2w(int nslices, tamx_context* ctx, tamx_worklist* wl,
                             const SplitParams& prm, hipStream_t s);
+// The grouped split GEMM for INT8 limbs with four-wave workgroups, 64 x 64
+// macro tiles on 32x32x32 blocks and a four-stage LDS ring of 32-k chunks
+// (tamx_split_i8.hip); plans its items and launches.
+int launch_split_gemm_i8_ring(int nslices, tamx_context* ctx, tamx_worklist* wl,
+                              const SplitParams& prm, hipStream_t s);
 
 }  // namespace split
 }  // namespace tamx

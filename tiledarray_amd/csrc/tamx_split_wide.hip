@@ -112,31 +112,48 @@ __global__ __launch_bounds__(kThreads, 1) void split_gemm_wide_kernel(SplitParam
     // load cursor: pair lp, chunk lk0; runs one chunk ahead of compute
     int lp = t_pb, lk0 = 0, lkp = 0;
     int64_t la_row = 0, lb_row = 0, la_plane = 0, lb_plane = 0, la_step = 0, lb_step = 0;
+    int64_t la_blk = 0, lb_blk = 0;
     auto set_pair = [&](int pi) {
       const auto& pp = tamx::scalar_ptr(p.pairs)[pi];
       const auto* A = tamx::scalar_ptr(p.ta) + uni(pp.a_tile);
       const auto* B = tamx::scalar_ptr(p.tb) + uni(pp.b_tile);
       lkp = uni(A->k_pad);
       // limb offsets in 16-byte units: plane stride, this thread's first
-      // staging row, and the 64-row step between its rows
+      // staging row, and the 64-row step between its rows (INT8: in the
+      // 32-k block layout, with the units per block)
       la_plane = uni64((int64_t)uni(A->rows_pad) * lkp) >> USH;
       lb_plane = uni64((int64_t)uni(B->rows_pad) * lkp) >> USH;
-      la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
-      lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
-      la_step = lb_step = (int64_t)(64 >> USH) * lkp;
+      if constexpr (I8) {
+        la_blk = 2 * uni(A->rows_pad);
+        lb_blk = 2 * uni(B->rows_pad);
+        la_row = i8_unit(uni64(A->limb_off) >> 4, 0, r0 + st_r, 0);
+        lb_row = i8_unit(uni64(B->limb_off) >> 4, 0, c0 + st_r, 0);
+        la_step = lb_step = 2 * 64;
+      } else {
+        la_row = (uni64(A->limb_off) + (int64_t)(r0 + st_r) * lkp) >> USH;
+        lb_row = (uni64(B->limb_off) + (int64_t)(c0 + st_r) * lkp) >> USH;
+        la_step = lb_step = (int64_t)(64 >> USH) * lkp;
+      }
     };
     if (lp < pend) set_pair(lp);
     // Once the cursor has passed the last pair the loads repeat its first
     // chunk, unused: every step issues the same loads, which keeps the wait
     // counts before the LDS stores exact.
     auto load_chunk = [&](bf8_t (&ra)[S][RA], bf8_t (&rb)[S][RB]) {
-      const int64_t q = (lk0 >> USH) + st_q;
+      int64_t qa, qb;  // unit st_q of the chunk at lk0 in the first staging row
+      if constexpr (I8) {
+        qa = la_row + ((lk0 / kI8Block) + (st_q >> 1)) * la_blk + (st_q & 1);
+        qb = lb_row + ((lk0 / kI8Block) + (st_q >> 1)) * lb_blk + (st_q & 1);
+      } else {
+        qa = la_row + (lk0 >> USH) + st_q;
+        qb = lb_row + (lk0 >> USH) + st_q;
+      }
 #pragma unroll
       for (int s = 0; s < S; ++s) {
 #pragma unroll
-        for (int u = 0; u < RA; ++u) ra[s][u] = p.LA[la_row + u * la_step + s * la_plane + q];
+        for (int u = 0; u < RA; ++u) ra[s][u] = p.LA[qa + u * la_step + s * la_plane];
 #pragma unroll
-        for (int u = 0; u < RB; ++u) rb[s][u] = p.LB[lb_row + u * lb_step + s * lb_plane + q];
+        for (int u = 0; u < RB; ++u) rb[s][u] = p.LB[qb + u * lb_step + s * lb_plane];
       }
       if (lp < pend) {
         lk0 += CK;
