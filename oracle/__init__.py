@@ -200,3 +200,62 @@ def contract(op_a, op_b, mext, next_, kext, A, a_off, B, b_off, c_off, c_size, a
                np.float32(pair_filter_threshold))
     del args
     return C, int(npairs)
+
+
+def arena_slab_cells(row_bounds, col_bounds):
+    """Cell ordinals (r * nc + c) in arena slab order: outer tiles row-major,
+    cells row-major inside each tile; and the prefix of cells per tile."""
+    rb, cb = np.asarray(row_bounds, np.int64), np.asarray(col_bounds, np.int64)
+    nc = int(cb[-1] - cb[0])
+    ids, pre = [], [0]
+    for tr in range(rb.size - 1):
+        for tc in range(cb.size - 1):
+            t = (np.arange(rb[tr], rb[tr + 1])[:, None] * nc +
+                 np.arange(cb[tc], cb[tc + 1])[None, :]).reshape(-1)
+            ids.append(t)
+            pre.append(pre[-1] + t.size)
+    return (np.concatenate(ids) if ids else np.zeros(0, np.int64)), np.asarray(pre, np.int64)
+
+
+def arena_contract(i_bounds, k_bounds, j_bounds, ea, ec, eb, A, a_cell_off, B, b_cell_off,
+                   c_cell_off, c_size, alpha=1.0, threshold=float(np.finfo(np.float32).eps)):
+    """Contraction of two arena tensors of tensors, C(i,j) = alpha sum_k
+    A(i,k) . B(k,j) over cells, cell (i,k) of A being ea[i] x ec[k] (ec = 1:
+    the inner outer product) and (k,j) of B ec[k] x eb[j], as TiledArray
+    runs it: outer-tile norms over the tiles' cells (tensor/tensor.h:3768-3776),
+    scaled by inverse cells per tile and screened (sparse_shape.h:150-217),
+    the outer-tile result estimate with cells per contracted tile as k sizes
+    (sparse_shape.h:1589-1681), then per non-zero result tile and per
+    non-zero operand tile pair the cell products in ascending k (the per-cell
+    contraction of arena_einsum.h:1803, which the strided regimes of
+    arena_einsum.h:1170-1800 compute in other groupings).  Result cells are
+    written at c_cell_off (nI x nJ, < 0 skipped).  Returns (C pool,
+    result-tile estimate, scaled A tile norms, scaled B tile norms)."""
+    ib, kb, jb = (np.asarray(x, np.int64) for x in (i_bounds, k_bounds, j_bounds))
+    ea, ec, eb = (np.asarray(x, np.int64) for x in (ea, ec, eb))
+    nI, nK, nJ = ib.size - 1, kb.size - 1, jb.size - 1
+    a_cell_off = np.asarray(a_cell_off, np.int64).reshape(ea.size, ec.size)
+    b_cell_off = np.asarray(b_cell_off, np.int64).reshape(ec.size, eb.size)
+    c_cell_off = np.asarray(c_cell_off, np.int64).reshape(ea.size, eb.size)
+    a_vol = (ea[:, None] * ec[None, :])
+    b_vol = (ec[:, None] * eb[None, :])
+    ida, pa = arena_slab_cells(ib, kb)
+    idb, pb = arena_slab_cells(kb, jb)
+    na = tot_tile_norms(A, pa, a_cell_off.reshape(-1)[ida], a_vol.reshape(-1)[ida])
+    nb = tot_tile_norms(B, pb, b_cell_off.reshape(-1)[idb], b_vol.reshape(-1)[idb])
+    sa, _ = scale_tile_norms(na.reshape(nI, nK), [np.diff(ib), np.diff(kb)], threshold)
+    sb, _ = scale_tile_norms(nb.reshape(nK, nJ), [np.diff(kb), np.diff(jb)], threshold)
+    est, _ = shape_gemm(sa, sb, 0, 0, nI, nJ, nK, np.diff(kb).astype(np.float32), abs(alpha),
+                        threshold)
+    # tile decisions replicated onto the cells
+    ti = np.repeat(np.arange(nI), np.diff(ib))
+    tk = np.repeat(np.arange(nK), np.diff(kb))
+    tj = np.repeat(np.arange(nJ), np.diff(jb))
+    a_on = sa[ti[:, None], tk[None, :]] >= threshold
+    b_on = sb[tk[:, None], tj[None, :]] >= threshold
+    c_on = est[ti[:, None], tj[None, :]] >= threshold
+    ao = np.where(a_on, a_cell_off, -1).reshape(-1)
+    bo = np.where(b_on, b_cell_off, -1).reshape(-1)
+    co = np.where(c_on, c_cell_off, -1).reshape(-1)
+    C, _ = contract(0, 0, ea, eb, ec, A, ao, B, bo, co, c_size, alpha=alpha)
+    return C, est, sa, sb
