@@ -35,8 +35,10 @@ PEAK_FP32_TFLOPS = 157.3
 # with the profile summary the counters come from; a line whose
 # configuration has no entry reports traffic null.
 TRAFFIC_BYTES = {
-    ("blocksparse", "f64", (64, 64, 16)): (3785074 * 2048 + 942487 * 1024,
+    ("blocksparse", "f64", (64, 64, 16)): (4668000 * 2048 + 1006000 * 1024,
                                            "profiles/grouped_gemm_f64_blocksparse16384.md"),
+    ("blocksparse", "i8x5", (64, 64, 16)): (2202000 * 2048 + 905900 * 1024,
+                                            "profiles/split_gemm_i8x5_blocksparse16384.md"),
 }
 
 
@@ -331,10 +333,9 @@ def main():
             "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
             "seed": info["seed"],
         },
-        "roofline": {
+        "roofline": ({
             "bound": "mfma",
-            "kernel": (f"grouped_gemm_kernel<double,{','.join(map(str, used_tile or [64, 64, 16]))}>"
-                       if args.precision == "f64" else "split_gemm_kernel"),
+            "kernel": f"grouped_gemm_kernel<double,{','.join(map(str, used_tile or [64, 64, 16]))}>",
             "achieved": round(achieved, 3),
             "peak": peak,
             "unit": "TFLOP/s",
@@ -342,7 +343,9 @@ def main():
             "traffic": tr_bytes,
             "traffic_source": tr_src,
             "avg_launch_ms": round(gemm_ms, 4),
-        },
+        } if args.precision == "f64" else dict(
+            split_roofline(args.precision, flops, gemm_ms), kernel="split GEMM",
+            traffic=tr_bytes, traffic_source=tr_src, avg_launch_ms=round(gemm_ms, 4))),
     }
     if sparse:
         out["config"]["near_threshold_estimates"] = near_threshold(ctx, c)
@@ -505,9 +508,11 @@ def bench_ladder(args):
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
                      "precision": main_prec, "seed": info["seed"],
                      "v_permute": "copy" if hasattr(c.B, "_perm_plan") else "index map"},
-                    {"bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                     "traffic": None, "avg_launch_ms": round(gemm_ms, 4)})
+                    ({"bound": "mfma", "achieved": round(achieved, 3),
+                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                      "traffic": None, "avg_launch_ms": round(gemm_ms, 4)} if main_prec == "f64"
+                     else dict(split_roofline(main_prec, flops, gemm_ms), traffic=None,
+                               avg_launch_ms=round(gemm_ms, 4))))
     if main_prec == "f64" and not hasattr(c.B, "_perm_plan"):
         # the permute kernel on V outside the step (into a scratch copy)
         Y = E.permute_array(ctx, V, list(c.plan.right_perm[:4]))
