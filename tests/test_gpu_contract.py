@@ -285,8 +285,9 @@ def test_permute_bit_exact_vs_oracle(ctx, rank):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_permute_large_and_mixed_planes(ctx, dtype):
-    # planes above 96 x 96 go through 32 x 32 tiles, smaller ones whole; one
-    # plan holds both kinds, with ragged edges on every tile
+    # planes above 96 x 96 elements move in blocks (one side or both split
+    # evenly), smaller ones whole; one plan holds both kinds, with ragged
+    # edges on every tile
     rng = np.random.default_rng(77)
     tr = TiledRange([TiledRange1([0, 130, 170, 301]), TiledRange1([0, 97, 100, 260]),
                      TiledRange1([0, 3, 5])])
@@ -296,6 +297,31 @@ def test_permute_large_and_mixed_planes(ctx, dtype):
         Y = E.permute_array(ctx, X, perm)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(Y.to_dense(), oracle.permute(X.to_dense(), perm))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("bt, dt", [([0, 13, 24], [0, 11, 22]), ([0, 45, 90], [0, 19, 40]),
+                                    ([0, 3, 150], [0, 7, 9])],
+                         ids=["short_rows", "long_rows", "skewed"])
+def test_permute_joined_rows(ctx, dtype, bt, dt):
+    # (a,c,b,d) -> (c,d,a,b), the ladder's permutation: consecutive a are
+    # consecutive items of one workgroup (tens of thousands of planes, so a
+    # workgroup takes several), and the cache line where the rows of a and
+    # a + 1 meet is written by both items together; odd row lengths put
+    # that line anywhere, and a row can lie inside one line
+    rng = np.random.default_rng(91)
+    tr = TiledRange([TiledRange1([0, 37, 100]), TiledRange1([0, 55, 100]),
+                     TiledRange1(bt), TiledRange1(dt)])
+    X = DistArray.allocate(tr, rng.random(tr.ntiles) < 0.85, dtype=dtype)
+    X.pool.copy_(torch.as_tensor(rng.uniform(-1, 1, X.pool.numel()), dtype=dtype))
+    Y = E.permute_array(ctx, X, [2, 0, 3, 1])
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(Y.to_dense(), oracle.permute(X.to_dense(), [2, 0, 3, 1]))
+    # a second execution into the same result leaves it unchanged
+    first = Y.pool.clone()
+    Y._perm_plan.run(X.pool, Y.pool)
+    torch.cuda.synchronize()
+    assert torch.equal(Y.pool, first)
 
 
 def test_ladder_permutes_and_result_transpose(ctx):

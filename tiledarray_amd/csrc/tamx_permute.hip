@@ -4,11 +4,14 @@
 // strides.  Consecutive source modes that stay consecutive in the result are
 // fused first, as detail::permute does (tensor/permute.h:51-95).  Two cases:
 //   - the fastest source mode stays fastest: contiguous runs are copied;
-//   - otherwise the (fastest source mode) x (mode that becomes fastest in the
-//     result) plane is moved through LDS so that both the loads and the
-//     stores are contiguous along a row: whole when it has at most 96 x 96
-//     elements, else in padded 32 x 32 tiles; every other mode is a batch
-//     index.
+//   - otherwise the (fastest source mode s) x (mode t that becomes fastest
+//     in the result) plane is moved through LDS so that both the loads and
+//     the stores are contiguous along a row: whole when it has at most
+//     96 x 96 elements, else in blocks of at most that many; every other
+//     mode is a batch index.
+// Batch indices run fastest-in-the-result first, so that a workgroup's
+// consecutive items write neighbouring result rows; the partial cache line
+// where two such rows meet is written by both items together.
 // Both are HBM-bandwidth bound: 2 * volume * sizeof(T) bytes per tile.
 #include <algorithm>
 #include <new>
@@ -19,13 +22,12 @@
 namespace {
 
 constexpr int kMaxRank = 8;
-constexpr int kTS = 32;       // transpose tile edge
-constexpr int kCopyRun = 2048;  // elements per copy item
-// a transposed plane of at most this many elements moves through LDS whole
-// (one item per plane, no partial edge tiles)
+constexpr int kCopyRun = 4096;  // elements per copy item (eight per thread)
+// a transposed block holds at most kPlaneMax elements; its LDS rows are
+// padded to an odd length, which needs at most kPlaneEdge more
 constexpr int kPlaneMax = 96 * 96;
+constexpr int kPlaneEdge = 96;
 constexpr int kPermThreads = 512;
-constexpr int kRowsPer = kPermThreads / kTS;  // tile rows a pass of the workgroup covers
 
 struct PermTile {
   int64_t src_off, dst_off;
@@ -36,7 +38,13 @@ struct PermTile {
   int32_t q;      // fused rank
   int32_t s_dim;  // fastest source mode (q - 1)
   int32_t t_dim;  // source mode that is fastest in the result
-  int32_t nbs, nbt;  // blocks along s and t (0 for a whole-plane tile)
+  int32_t bls, blt;  // block extents along s and t
+  int32_t nbs, nbt;  // blocks along s and t
+  // batch modes (every mode but s and t; but s for a copy), fastest in the
+  // result first: consecutive items of one workgroup write neighbouring
+  // result rows, so the cache lines two rows share are filled in one L2
+  int32_t nbatch;
+  int32_t border[kMaxRank];
 };
 
 }  // namespace
@@ -61,144 +69,234 @@ __device__ inline int64_t find_tile(const int64_t* prefix, int64_t n, int64_t it
   return lo;
 }
 
-// `plane` is the workgroup's LDS staging area (a whole plane, or one padded
-// 32 x 32 tile at its start).
-template <typename T>
-__device__ inline void permute_item(const T* __restrict__ src, T* __restrict__ dst,
-                                    const PermTile& t, int local, T* plane) {
-  const int q = t.q, sd = t.s_dim, td = t.t_dim;
+// Barrier for the LDS staging area only: this wave's LDS reads and writes
+// have completed, its global stores may still be in flight (a
+// __syncthreads() would wait for them too, draining the workgroup's stores
+// at every item).
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-  if (sd == td) {
-    // contiguous-run copy: item = (other modes, run block)
-    const int nruns = (t.ext[sd] + kCopyRun - 1) / kCopyRun;
-    const int rb = local % nruns;
-    local /= nruns;
-    int64_t so = t.src_off, doff = t.dst_off;
-    for (int d = q - 1; d >= 0; --d) {
-      if (d == sd) continue;
-      const int x = local % t.ext[d];
-      local /= t.ext[d];
-      so += x * t.sstride[d];
-      doff += x * t.dstride[d];
-    }
-    const int b0 = rb * kCopyRun;
-    const int n = min(kCopyRun, t.ext[sd] - b0);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) dst[doff + b0 + x] = src[so + b0 + x];
-    return;
-  }
-
-  if (t.nbs == 0) {
-    // whole plane: ns x nt elements, LDS laid out [t][s] with odd stride
-    const int ns = t.ext[sd], nt = t.ext[td];
-    int64_t so = t.src_off, doff = t.dst_off;
-    for (int d = q - 1; d >= 0; --d) {
-      if (d == sd || d == td) continue;
-      const int x = local % t.ext[d];
-      local /= t.ext[d];
-      so += x * t.sstride[d];
-      doff += x * t.dstride[d];
-    }
-    T* pl = plane;
-    const int ld = ns | 1;
-    const int n = ns * nt;
-    // element e = tid + i * blockDim of the plane is (e / w, e % w) for a row
-    // length w; one division per item, then each step of blockDim elements
-    // advances by (blockDim / w, blockDim % w) with one carry, and four
-    // elements per thread are loaded before any is stored
-    constexpr int U = 4;
-    {
-      const int qs = blockDim.x / ns, rs = blockDim.x - qs * ns;
-      int y = threadIdx.x / ns, x = threadIdx.x - y * ns;  // t index, s index
-      const T* __restrict__ sp = src + so;
-      const int64_t st = t.sstride[td];
-      for (int e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
-        T v[U];
-        int yy[U], xx[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          yy[u] = y;
-          xx[u] = x;
-          v[u] = e0 + u * (int)blockDim.x < n ? sp[(int64_t)y * st + x] : T(0);
-          x += rs;
-          y += qs;
-          if (x >= ns) {
-            x -= ns;
-            ++y;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (e0 + u * (int)blockDim.x < n) pl[yy[u] * ld + xx[u]] = v[u];
-      }
-    }
-    __syncthreads();
-    {
-      const int qt = blockDim.x / nt, rt = blockDim.x - qt * nt;
-      int y = threadIdx.x / nt, x = threadIdx.x - y * nt;  // s index, t index
-      T* __restrict__ dp = dst + doff;
-      const int64_t st = t.dstride[sd];
-      for (int e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (e0 + u * (int)blockDim.x < n) dp[(int64_t)y * st + x] = pl[x * ld + y];
-          x += rt;
-          y += qt;
-          if (x >= nt) {
-            x -= nt;
-            ++y;
-          }
-        }
-      }
-    }
-    return;
-  }
-
-  const int bs = local % t.nbs;
-  local /= t.nbs;
-  const int bt = local % t.nbt;
-  local /= t.nbt;
-  int64_t so = t.src_off, doff = t.dst_off;
-  for (int d = q - 1; d >= 0; --d) {
-    if (d == sd || d == td) continue;
+// source and result offsets of item `local`'s batch indices
+__device__ inline void batch_offsets(const PermTile& t, int local, int64_t& so, int64_t& doff) {
+  so = t.src_off;
+  doff = t.dst_off;
+  for (int i = 0; i < t.nbatch; ++i) {
+    const int d = t.border[i];
     const int x = local % t.ext[d];
     local /= t.ext[d];
     so += x * t.sstride[d];
     doff += x * t.dstride[d];
   }
-  const int s0 = bs * kTS, t0 = bt * kTS;
-  const int ns = min(kTS, t.ext[sd] - s0), nt = min(kTS, t.ext[td] - t0);
-  const int tx = threadIdx.x % kTS, ty = threadIdx.x / kTS;  // 32 x kRowsPer
-  T (*lds)[kTS + 1] = reinterpret_cast<T (*)[kTS + 1]>(plane);
-  // load: rows along t, contiguous along s (source stride 1); all of a
-  // thread's loads are issued before the LDS writes
-  T v[kTS / kRowsPer];
+}
+
+// contiguous-run copy: item = (run block, other modes); every element a
+// thread moves is loaded before any is stored
+template <typename T>
+__device__ inline void copy_item(const T* __restrict__ src, T* __restrict__ dst,
+                                 const PermTile& t, int local) {
+  const int sd = t.s_dim;
+  const int nruns = (t.ext[sd] + kCopyRun - 1) / kCopyRun;
+  const int rb = local % nruns;
+  int64_t so, doff;
+  batch_offsets(t, local / nruns, so, doff);
+  const int b0 = rb * kCopyRun;
+  const int n = min(kCopyRun, t.ext[sd] - b0);
+  const T* __restrict__ sp = src + so + b0;
+  T* __restrict__ dp = dst + doff + b0;
+  constexpr int U = kCopyRun / kPermThreads;
+  T v[U];
 #pragma unroll
-  for (int k = 0; k < kTS / kRowsPer; ++k) {
-    const int y = ty + kRowsPer * k;
-    v[k] = (tx < ns && y < nt) ? src[so + (int64_t)(t0 + y) * t.sstride[td] + s0 + tx] : T(0);
+  for (int u = 0; u < U; ++u) {
+    const int x = threadIdx.x + u * kPermThreads;
+    v[u] = x < n ? sp[x] : T(0);
   }
 #pragma unroll
-  for (int k = 0; k < kTS / kRowsPer; ++k) lds[ty + kRowsPer * k][tx] = v[k];
-  __syncthreads();
-  // store: rows along s, contiguous along t (result stride 1); the caller's
-  // barrier after the item protects the LDS tile before its next reuse
+  for (int u = 0; u < U; ++u) {
+    const int x = threadIdx.x + u * kPermThreads;
+    if (x < n) dp[x] = v[u];
+  }
+}
+
+// One block of the (s, t) plane of a transposed tile: ns x nt elements
+// (s fastest in the source, t fastest in the result), at most kPlaneMax.
+struct Block {
+  int64_t so, doff;  // element offsets of the block's first element
+  int ns, nt;
+};
+
+__device__ inline int64_t uniform(int64_t x) {
+  const uint64_t u = (uint64_t)x;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// item `local` = (s block, t block, batch indices)
+__device__ inline Block block_of(const PermTile& t, int local) {
+  const int bs = local % t.nbs;
+  local /= t.nbs;
+  const int bt = local % t.nbt;
+  local /= t.nbt;
+  Block b;
+  batch_offsets(t, local, b.so, b.doff);
+  const int s0 = bs * t.bls, t0 = bt * t.blt;
+  b.ns = min(t.bls, t.ext[t.s_dim] - s0);
+  b.nt = min(t.blt, t.ext[t.t_dim] - t0);
+  b.so += s0 + (int64_t)t0 * t.sstride[t.t_dim];
+  b.doff += t0 + (int64_t)s0 * t.dstride[t.s_dim];
+  // the block is the same for every lane: keep it in scalar registers
+  b.so = uniform(b.so);
+  b.doff = uniform(b.doff);
+  b.ns = __builtin_amdgcn_readfirstlane(b.ns);
+  b.nt = __builtin_amdgcn_readfirstlane(b.nt);
+  return b;
+}
+
+// Element e = tid + u * blockDim of an ns x nt block is row e / ns, column
+// e % ns: one division, then each step advances with one carry.  Every
+// element a thread moves is loaded into registers at once (the most bytes
+// in flight).
+constexpr int kPlaneRegs = (kPlaneMax + kPermThreads - 1) / kPermThreads;
+
+template <typename T>
+__device__ inline void block_load(const T* __restrict__ src, const PermTile& t, const Block& b,
+                                  T (&v)[kPlaneRegs]) {
+  const int ns = b.ns, n = ns * b.nt;
+  const int qs = blockDim.x / ns, rs = blockDim.x - qs * ns;
+  int y = threadIdx.x / ns, x = threadIdx.x - y * ns;
+  // each pass recomputes its indices: shared across passes they would
+  // hold two registers per element for the whole item
+  asm volatile("" : "+v"(x), "+v"(y));
+  // offsets inside one tile fit 32 bits (tile volumes are below 2^31)
+  const T* __restrict__ sp = src + b.so;
+  const uint32_t st = (uint32_t)t.sstride[t.t_dim];
 #pragma unroll
-  for (int k = 0; k < kTS / kRowsPer; ++k) {
-    const int y = ty + kRowsPer * k;
-    if (tx < nt && y < ns) dst[doff + (int64_t)(s0 + y) * t.dstride[sd] + t0 + tx] = lds[tx][y];
+  for (int u = 0; u < kPlaneRegs; ++u) {
+    v[u] = threadIdx.x + u * (int)blockDim.x < n ? sp[(uint32_t)y * st + (uint32_t)x] : T(0);
+    x += rs;
+    y += qs;
+    if (x >= ns) {
+      x -= ns;
+      ++y;
+    }
+  }
+}
+
+// the loaded elements into LDS, laid out [t][s] with an odd row stride
+template <typename T>
+__device__ inline void block_to_lds(const Block& b, const T (&v)[kPlaneRegs], T* pl) {
+  const int ns = b.ns, n = ns * b.nt;
+  const int ld = ns | 1;
+  const int qs = blockDim.x / ns, rs = blockDim.x - qs * ns;
+  int y = threadIdx.x / ns, x = threadIdx.x - y * ns;
+  // each pass recomputes its indices: shared across passes they would
+  // hold two registers per element for the whole item
+  asm volatile("" : "+v"(x), "+v"(y));
+#pragma unroll
+  for (int u = 0; u < kPlaneRegs; ++u) {
+    if (threadIdx.x + u * (int)blockDim.x < n) pl[y * ld + x] = v[u];
+    x += rs;
+    y += qs;
+    if (x >= ns) {
+      x -= ns;
+      ++y;
+    }
+  }
+}
+
+// Cache-line ends of a result row of nt elements whose first element lies
+// at byte address `lo` (low 32 bits suffice): the elements before its first
+// line boundary (head) and after its last (tail).
+template <typename T>
+__device__ inline int head_len(uint32_t lo, int nt) {
+  return min(nt, (int)(((128u - (lo & 127u)) & 127u) / sizeof(T)));
+}
+template <typename T>
+__device__ inline int tail_len(uint32_t lo, int nt) {
+  return (int)(((lo + (uint32_t)nt * sizeof(T)) & 127u) / sizeof(T));
+}
+
+// whether every result row of the block starts and ends on a cache line
+template <typename T>
+__device__ inline bool lines_whole(const T* dst, const PermTile& t, const Block& b) {
+  const uint32_t lo = (uint32_t)(uintptr_t)(dst + b.doff);
+  return ((lo | ((uint32_t)t.dstride[t.s_dim] * (uint32_t)sizeof(T)) |
+           ((uint32_t)b.nt * (uint32_t)sizeof(T))) & 127u) == 0;
+}
+
+// The block from LDS to the result, rows along the result's fastest mode:
+// columns [lo, hi) of every row, where lo skips the head when the previous
+// item wrote it (`skip_head`) and hi the tail when the caller writes it
+// later (`skip_tail`); `tail_only` writes just those tails.
+template <typename T>
+__device__ inline void block_store(T* __restrict__ dst, const PermTile& t, const Block& b,
+                                   const T* pl, bool skip_head, bool skip_tail, bool tail_only) {
+  const int ns = b.ns, nt = b.nt, n = ns * nt;
+  const int ld = ns | 1;
+  const int qt = blockDim.x / nt, rt = blockDim.x - qt * nt;
+  int y = threadIdx.x / nt, x = threadIdx.x - y * nt;  // s index, t index
+  const uint32_t st = (uint32_t)t.dstride[t.s_dim];
+  T* __restrict__ dp = dst + b.doff;
+  const uint32_t lo = (uint32_t)(uintptr_t)dp;
+  // a loop rather than a full unroll: it indexes no register array, and
+  // the next block's loads hold registers meanwhile
+#pragma unroll 6
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t row = (uint32_t)y * st;  // offsets inside a tile fit 32 bits
+    const int h = skip_head ? head_len<T>(lo + row * (uint32_t)sizeof(T), nt) : 0;
+    const int e = skip_tail ? max(h, nt - tail_len<T>(lo + row * (uint32_t)sizeof(T), nt)) : nt;
+    if (tail_only ? x >= e : (x >= h && x < e)) dp[row + (uint32_t)x] = pl[x * ld + y];
+    x += rt;
+    y += qt;
+    if (x >= nt) {
+      x -= nt;
+      ++y;
+    }
+  }
+}
+
+// The heads of the next block's result rows, from the registers its loads
+// landed in (the load order: element e is s index e % ns, t index e / ns).
+template <typename T>
+__device__ inline void block_store_heads(T* __restrict__ dst, const PermTile& t, const Block& b,
+                                         const T (&v)[kPlaneRegs]) {
+  const int ns = b.ns, nt = b.nt, n = ns * nt;
+  const int qs = blockDim.x / ns, rs = blockDim.x - qs * ns;
+  int y = threadIdx.x / ns, x = threadIdx.x - y * ns;  // t index, s index
+  // each pass recomputes its indices: shared across passes they would
+  // hold two registers per element for the whole item
+  asm volatile("" : "+v"(x), "+v"(y));
+  const uint32_t st = (uint32_t)t.dstride[t.s_dim];
+  T* __restrict__ dp = dst + b.doff;
+  const uint32_t lo = (uint32_t)(uintptr_t)dp;
+#pragma unroll
+  for (int u = 0; u < kPlaneRegs; ++u) {
+    // a head has fewer elements than a cache line: t index below 128 / sizeof(T)
+    if (u * (int)blockDim.x >= (128 / (int)sizeof(T)) * ns) break;
+    if (threadIdx.x + u * (int)blockDim.x < n) {
+      const uint32_t row = (uint32_t)x * st;
+      if (y < head_len<T>(lo + row * (uint32_t)sizeof(T), nt)) dp[row + (uint32_t)y] = v[u];
+    }
+    x += rs;
+    y += qs;
+    if (x >= ns) {
+      x -= ns;
+      ++y;
+    }
   }
 }
 
 // Each workgroup takes a contiguous range of items, so consecutive items
 // mostly belong to one tile: the tile is found by binary search once and
-// then followed forward.
+// then followed forward.  Blocks of one tile are pipelined: the next block
+// is loaded while this one is stored.
 template <typename T>
 __global__ __launch_bounds__(kPermThreads) void permute_kernel(
     const T* __restrict__ src, T* __restrict__ dst,
     const PermTile* __restrict__ tiles, const int64_t* __restrict__ prefix,
     int64_t ntiles) {
-  static_assert(kPlaneMax + 96 >= kTS * (kTS + 1), "staging area holds a tile");
-  __shared__ T plane[kPlaneMax + 96];  // rows padded to an odd length
+  static_assert(kPlaneRegs * kPermThreads >= kPlaneMax, "registers hold a block");
+  static_assert(kCopyRun % kPermThreads == 0, "a copy item is whole passes of the workgroup");
+  __shared__ T plane[kPlaneMax + kPlaneEdge];  // rows padded to an odd length
   const int64_t nitems = prefix[ntiles];
   const int64_t per = (nitems + gridDim.x - 1) / gridDim.x;
   const int64_t ib = min<int64_t>(nitems, (int64_t)blockIdx.x * per);
@@ -206,17 +304,78 @@ __global__ __launch_bounds__(kPermThreads) void permute_kernel(
   if (ib >= ie) return;
   int64_t ti = find_tile(prefix, ntiles, ib);
   int64_t tnext = prefix[ti + 1];
+  T v[kPlaneRegs];
+  bool have = false;  // v holds the block of `item`
+  bool prev = false;  // the previous item wrote this block's row heads
+  Block cur{}, nxt{};
   for (int64_t item = ib; item < ie; ++item) {
     while (item >= tnext) {
       ++ti;
       tnext = prefix[ti + 1];
     }
-    permute_item<T>(src, dst, tiles[ti], (int)(item - prefix[ti]), plane);
-    __syncthreads();
+    const PermTile& t = tiles[ti];
+    const int local = (int)(item - prefix[ti]);
+    if (t.s_dim == t.t_dim) {
+      copy_item<T>(src, dst, t, local);
+      continue;
+    }
+    if (!have) {
+      cur = block_of(t, local);
+      block_load<T>(src, t, cur, v);
+    } else {
+      cur = nxt;
+    }
+    block_to_lds<T>(cur, v, plane);
+    lds_barrier();
+    have = item + 1 < ie && item + 1 < tnext;
+    if (have) {
+      nxt = block_of(t, local + 1);  // next block, same tile
+      block_load<T>(src, t, nxt, v);
+    }
+    // When the next block's rows continue this one's in memory (the same
+    // shape, its result starting where this block's first row ends, as for
+    // consecutive indices of the batch mode just slower than t), each
+    // row's last, partial cache line is shared with the next block's row:
+    // this block's row tails and the next block's row heads are then
+    // written back to back, so both parts of the line reach the L2
+    // together instead of an item apart.
+    const bool join = have && nxt.ns == cur.ns && nxt.nt == cur.nt &&
+                      nxt.doff == cur.doff + cur.nt && !lines_whole<T>(dst, t, cur);
+    block_store<T>(dst, t, cur, plane, prev, join, false);
+    if (join) {
+      block_store<T>(dst, t, cur, plane, prev, true, true);
+      block_store_heads<T>(dst, t, nxt, v);
+    }
+    prev = join;
+    lds_barrier();
   }
 }
 
 }  // namespace
+
+// Blocks of an ns x nt plane: the whole plane when it fits the staging
+// area, else the long side (or both) split evenly so that every block does.
+static bool split_plane(int64_t ns, int64_t nt, PermTile& t) {
+  auto fits = [](int64_t bs, int64_t bt) {
+    return bs * bt <= kPlaneMax && (bs | 1) * bt <= kPlaneMax + kPlaneEdge;
+  };
+  int64_t nbs = 1, nbt = 1;
+  if (!fits(ns, nt)) {
+    if (ns <= 96) {
+      nbt = (nt + kPlaneMax / (ns | 1) - 1) / (kPlaneMax / (ns | 1));
+    } else if (nt <= 96) {
+      nbs = (ns + kPlaneMax / nt - 1) / (kPlaneMax / nt);
+    } else {
+      nbs = (ns + 95) / 96;
+      nbt = (nt + 95) / 96;
+    }
+  }
+  t.bls = (int32_t)((ns + nbs - 1) / nbs);
+  t.blt = (int32_t)((nt + nbt - 1) / nbt);
+  t.nbs = (int32_t)((ns + t.bls - 1) / t.bls);
+  t.nbt = (int32_t)((nt + t.blt - 1) / t.blt);
+  return fits(t.bls, t.blt);
+}
 
 static unsigned permute_grid(const tamx_context* ctx, const tamx_permute_plan* plan) {
   // grid-stride over items: enough workgroups to fill every CU several times
@@ -314,6 +473,11 @@ int tamx_permute_plan_create(tamx_context* ctx, int rank, const int32_t* perm,
     t.t_dim = q - 1;
     for (int d = 0; d < q; ++d)
       if (t.dstride[d] == 1) t.t_dim = d;
+    t.nbatch = 0;
+    for (int d = 0; d < q; ++d)
+      if (d != t.s_dim && d != t.t_dim) t.border[t.nbatch++] = d;
+    std::sort(t.border, t.border + t.nbatch,
+              [&](int x, int y) { return t.dstride[x] < t.dstride[y]; });
     int64_t nitems = 0;
     if (vol == 0) {
       nitems = 0;
@@ -322,11 +486,9 @@ int tamx_permute_plan_create(tamx_context* ctx, int rank, const int32_t* perm,
       for (int d = 0; d < q; ++d)
         if (d != t.s_dim) nitems *= t.ext[d];
     } else {
-      const bool whole = (int64_t)(t.ext[t.s_dim] | 1) * t.ext[t.t_dim] <= kPlaneMax + 96 &&
-                         t.ext[t.s_dim] <= 96 && t.ext[t.t_dim] <= 96;
-      t.nbs = whole ? 0 : (t.ext[t.s_dim] + kTS - 1) / kTS;
-      t.nbt = whole ? 0 : (t.ext[t.t_dim] + kTS - 1) / kTS;
-      nitems = whole ? 1 : (int64_t)t.nbs * t.nbt;
+      TAMX_REQUIRE(split_plane(t.ext[t.s_dim], t.ext[t.t_dim], t),
+                   "permute block exceeds the staging area");
+      nitems = (int64_t)t.nbs * t.nbt;
       for (int d = 0; d < q; ++d)
         if (d != t.s_dim && d != t.t_dim) nitems *= t.ext[d];
     }
