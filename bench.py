@@ -269,19 +269,27 @@ def main():
     maxabs = {id(X): torch.zeros(X.trange.ntiles, dtype=torch.float64, device=X.pool.device)
               for X in (A, B)}
 
-    def screening(with_max):
-        for X in (A, B):
-            E.tile_norms(ctx, X, out=X.shape_norms, maxabs=maxabs[id(X)] if with_max else None)
-            E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
+    def screen(X, with_max):
+        E.tile_norms(ctx, X, out=X.shape_norms, maxabs=maxabs[id(X)] if with_max else None)
+        E.scale_norms(ctx, X.shape_norms, X.trange, thr, sizes=sizes[id(X)])
 
     def make_step(c):
         def step(events):
-            if sparse:
-                screening(bool(c.nslices))
-            if c.nslices:
-                # the limb split overlapped with the shape estimate and work list
-                c.split_and_plan(maxabs=(maxabs[id(A)], maxabs[id(B)]) if sparse else None)
+            if sparse and c.nslices:
+                # each operand's limb split starts on a side stream once its
+                # norm pass (which gives the tile maxima) is enqueued: A's
+                # split runs beside B's norms, both beside the shape
+                # estimate and the work list
+                for w, X in enumerate((A, B)):
+                    screen(X, True)
+                    c.split_operand_async(w, maxabs[id(X)])
+                c.plan_and_join()
+            elif c.nslices:
+                c.split_and_plan()
             else:
+                if sparse:
+                    for X in (A, B):
+                        screen(X, False)
                 c.shape_step()
                 c.worklist_step()
             if events is not None:

@@ -83,6 +83,37 @@ def test_norms_with_max_and_premax_split(ctx):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
+@pytest.mark.parametrize("precision", ["bf16x4", "i8x5"])
+def test_split_per_operand_overlap_equals_sequential(ctx, precision):
+    # each operand's split enqueued on the side stream right after its own
+    # norm pass (the other operand's norms, the shape estimate and the work
+    # list run beside it) gives the limbs and the result of the sequential
+    # step bit for bit, step after step
+    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=21)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision=precision)
+    c.run()
+    torch.cuda.synchronize()
+    limbs = [c.a_split.limbs.clone(), c.b_split.limbs.clone()]
+    C0 = c.C.clone()
+    mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
+    mx = {id(X): torch.zeros(X.trange.ntiles, dtype=torch.float64, device="cuda") for X in (A, B)}
+    for _ in range(3):
+        for sp in (c.a_split, c.b_split):
+            sp.limbs.zero_()
+        c.C.zero_()
+        for w, X in enumerate((A, B)):
+            E.tile_norms(ctx, X, out=X.shape_norms, maxabs=mx[id(X)])
+            E.scale_norms(ctx, X.shape_norms, X.trange, 1e-8)
+            c.split_operand_async(w, mx[id(X)])
+        c.plan_and_join()
+        c.gemm_step()
+        torch.cuda.synchronize()
+        assert torch.equal(c.a_split.limbs, limbs[0]) and torch.equal(c.b_split.limbs, limbs[1])
+        assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
+
+
 @pytest.mark.parametrize("tile", [(128, 64), (64, 128)])
 @pytest.mark.parametrize("precision", ["bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))

@@ -631,6 +631,39 @@ class Contraction:
                 call("tamx_split_f64_premax", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
                      m.data_ptr(), sp.limbs.data_ptr(), self.ctx.stream())
 
+    def _side_stream(self):
+        if self._side is None:
+            self._side = torch.cuda.Stream(self.ctx.device)  # default priority
+        return self._side
+
+    def split_operand_async(self, which, maxabs=None):
+        """Limb split of one operand (0: A, 1: B) on a side stream, after
+        the work already enqueued on the current stream (typically that
+        operand's norm pass, which gives `maxabs`), so that one operand's
+        split runs while the other's norms are computed.  plan_and_join()
+        runs the planning steps and makes the current stream wait for the
+        splits."""
+        X, sp = ((self.A, self.a_split), (self.B, self.b_split))[which]
+        main = torch.cuda.current_stream(self.ctx.device)
+        side = self._side_stream()
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            if maxabs is None:
+                call("tamx_split_f64", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
+                     sp.limbs.data_ptr(), self.ctx.stream())
+            else:
+                call("tamx_split_f64_premax", self.ctx.ptr, sp.ptr, X.pool.data_ptr(),
+                     maxabs.data_ptr(), sp.limbs.data_ptr(), self.ctx.stream())
+        if maxabs is not None:
+            maxabs.record_stream(side)
+
+    def plan_and_join(self):
+        """Shape estimate and work list on the current stream, then wait for
+        the limb splits enqueued by split_operand_async."""
+        self.shape_step()
+        self.worklist_step()
+        torch.cuda.current_stream(self.ctx.device).wait_stream(self._side_stream())
+
     def split_and_plan(self, maxabs=None):
         """The limb split of both operands and the planning steps (shape
         estimate, work list), which do not depend on each other: the split
@@ -639,9 +672,7 @@ class Contraction:
         enqueued after it sees the limbs."""
         dev = self.ctx.device
         main = torch.cuda.current_stream(dev)
-        if self._side is None:
-            self._side = torch.cuda.Stream(dev)  # default priority
-        side = self._side
+        side = self._side_stream()
         side.wait_stream(main)
         with torch.cuda.stream(side):
             self.split_step(maxabs)
