@@ -506,7 +506,8 @@ int tamx_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl, double alpha,
                             double* C, void* stream);
 /* The same with a macro tile: bm x bn of 64 x 64 (the default, any limb
  * count), 128 x 64 or 64 x 128 (2 to 4 BF16 or 3 to 4 INT8 limbs; one wave
- * per SIMD, a quarter fewer limb bytes loaded per result).  For INT8 limbs
+ * per SIMD, a quarter fewer limb bytes loaded per result), 128 x 64 for 5 or
+ * 6 INT8 limbs (eight waves of 32 x 32 in one workgroup per CU).  For INT8 limbs
  * at 64 x 64, bk picks the kernel: 32, each wave's 32 x 32 quadrant on the
  * INT8 MFMA 32x32x32 with 32-k chunks through a four-stage LDS ring (three
  * chunks in flight; chunks of k padding skipped); 64, the same blocks with

@@ -156,6 +156,49 @@ def test_split_wide_tiles_match_default(ctx, tile, precision):
     assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
 
 
+@pytest.mark.parametrize("precision", ["i8x5", "i8x6"])
+@pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))
+def test_split_i8_eight_wave_integer_exact(ctx, precision, op_a, op_b):
+    # 128 x 64 macro tiles from five INT8 limbs on (eight waves, B staged by
+    # four of them) on tiles wider and narrower than one macro tile, ragged
+    # in both directions, and taller than 128 rows
+    nslices = int(precision.split("x")[-1])
+    rng = np.random.default_rng(1300 + nslices * 4 + op_a * 2 + op_b)
+    trm = TiledRange1([0, 40, 170, 300, 301])
+    trk = TiledRange1([0, 17, 64, 150, 151, 300])
+    trn = TiledRange1([0, 96, 97, 250])
+    A, B = matrix_ops_arrays(rng, trm, trk, trn, op_a, op_b, torch.float64, True, fill=0.8)
+    la = "k,m" if op_a else "m,k"
+    lb = "n,k" if op_b else "k,n"
+    c = E.Contraction(ctx, "m,n", A, la, B, lb, alpha=2.0, precision=precision,
+                      tile=(128, 64))
+    c.run()
+    torch.cuda.synchronize()
+    Cref, npairs, c_off = oracle_contract(c, A, B)
+    mask = c_off_mask(c_off, c)
+    np.testing.assert_array_equal(c.C.cpu().numpy()[mask], Cref[mask])
+
+
+@pytest.mark.parametrize("precision", ["i8x5", "i8x6"])
+def test_split_i8_eight_wave_matches_default(ctx, precision):
+    # same limbs, same pair order, same levels and folds: the eight-wave
+    # kernel's result equals the default kernel's bit for bit on random data
+    A, B, _ = W.blocksparse_gemm(n=1500, wlo=24, whi=96, seed=23)
+    E.make_sparse(ctx, A, 1e-8)
+    E.make_sparse(ctx, B, 1e-8)
+    c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision=precision)
+    c.run()
+    torch.cuda.synchronize()
+    C0 = c.C.clone()
+    for _ in range(2):
+        c.set_tile((128, 64))
+        c.C.zero_()
+        c.gemm_step()
+        torch.cuda.synchronize()
+        mask = torch.as_tensor(c_off_mask(c.c_off.cpu().numpy(), c), device="cuda")
+        assert torch.equal(c.C[:mask.numel()][mask], C0[:mask.numel()][mask])
+
+
 @pytest.mark.parametrize("bk", [32, 64, 128])
 @pytest.mark.parametrize("precision", ["i8x3", "i8x4", "i8x5"])
 @pytest.mark.parametrize("op_a,op_b", list(itertools.product([0, 1], [0, 1])))

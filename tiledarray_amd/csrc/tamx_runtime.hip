@@ -237,7 +237,8 @@ int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
   // 64 x 128 for 2 to 4 BF16 or 3 to 4 INT8 limbs; for INT8 limbs at 64 x 64
   // bk picks the kernel: 16, the 16x16x64 MFMA; 32, the 32x32x32 blocks with
   // a four-stage LDS ring of 32-k chunks; 64, the 32x32x32 blocks with two
-  // 64-k buffers; 128, the two-wave kernel (4 or 5 limbs)
+  // 64-k buffers; 128, the two-wave kernel (4 or 5 limbs); from five INT8
+  // limbs on, 128 x 64 is the eight-wave 16x16x64 kernel
   std::vector<tamx_gemm_config> cands{{64, 64, 32, 0}};
   if (limb_type == TAMX_LIMB_BF16 ? nslices <= 4 : (nslices >= 3 && nslices <= 4)) {
     cands.push_back({128, 64, 32, 0});
@@ -249,6 +250,7 @@ int tamx_autotune_grouped_gemm_split(tamx_context* ctx, tamx_worklist* wl,
   }
   if (limb_type == TAMX_LIMB_I8 && (nslices == 4 || nslices == 5))
     cands.push_back({64, 64, 128, 0});
+  if (limb_type == TAMX_LIMB_I8 && nslices >= 5) cands.push_back({128, 64, 16, 0});
   *best = cands[0];
   if (best_ms) *best_ms = 0.0f;
   if (sum.ntasks == 0) return TAMX_OK;

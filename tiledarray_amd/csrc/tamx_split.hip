@@ -370,9 +370,15 @@ void launch_split(int grid, hipStream_t s, const double* pool, const tamx_split_
 // MFMA 32x32x32 instead of four of the 16x16x64 (the same products and
 // levels; on this part the 32x32x32 does twice the work in about the same
 // issue time).
-template <int S, bool I8, bool M32 = false>
-__global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitParams p) {
+// WM: waves along the rows of the macro tile, 2 (64 x 64, four waves) or 4
+// (128 x 64, eight waves in one workgroup per CU: every B limb staged into
+// LDS feeds twice the rows, a quarter fewer limb bytes per product; INT8
+// limbs from three on, whose chunks are loaded straight into LDS).
+template <int S, bool I8, bool M32 = false, int WM = 2>
+__global__ __launch_bounds__(128 * WM, I8 ? 2 : 1) void split_gemm_kernel(SplitParams p) {
   static_assert(!M32 || I8, "the 32x32x32 blocks are INT8 only");
+  static_assert(WM == 2 || (WM == 4 && I8 && !M32 && S > 2), "eight waves: INT8 16x16 blocks, 3+ limbs");
+  constexpr int BM = 32 * WM;  // macro tile rows
   constexpr int NG = I8 ? S : (S >= 3 ? 3 : S);  // BF16: P_0, P_1, P_T; INT8: levels
   constexpr int CK = I8 ? SBK8 : SBK;            // k per chunk
   constexpr int USH = I8 ? 4 : 3;                // log2 limbs per 16-byte unit
@@ -383,13 +389,13 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
   // (one variable per buffer: the compiler then knows that loads landing in
   // one buffer do not alias reads of the other, and waits for them only
   // where the buffer they fill is read)
-  __shared__ bf8_t sA0[S][SBM * 4], sA1[S][SBM * 4];
+  __shared__ bf8_t sA0[S][BM * 4], sA1[S][BM * 4];
   __shared__ bf8_t sB0[S][SBN * 4], sB1[S][SBN * 4];
   int* s_item = reinterpret_cast<int*>(&sA0[0][0]);
   using zero = std::integral_constant<int, 0>;
   using one = std::integral_constant<int, 1>;
   using two = std::integral_constant<int, 2>;
-  auto bufA = [&](auto B) -> bf8_t(&)[S][SBM * 4] {
+  auto bufA = [&](auto B) -> bf8_t(&)[S][BM * 4] {
     if constexpr (decltype(B)::value == 0) return sA0; else return sA1;
   };
   auto bufB = [&](auto B) -> bf8_t(&)[S][SBN * 4] {
@@ -449,8 +455,8 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
     const int t_flags = uni(tp->flags);
     const int local = item - uni(p.prefix[lo]);
     const int tiles_n = (t_n + SBN - 1) / SBN;
-    const int r0 = (local / tiles_n) * SBM, c0 = (local % tiles_n) * SBN;
-    const int mrem = min(SBM, t_m - r0), nrem = min(SBN, t_n - c0);
+    const int r0 = (local / tiles_n) * BM, c0 = (local % tiles_n) * SBN;
+    const int mrem = min(BM, t_m - r0), nrem = min(SBN, t_n - c0);
     const bool wave_active = (wr < mrem) && (wc < nrem);
 
     // 16 x 16 blocks (acc, lvl) or one 32 x 32 block (acc32, lvl32)
@@ -548,9 +554,11 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
         __builtin_amdgcn_global_load_lds(
             (const __attribute__((address_space(1))) void*)(p.LA + ua + s * la_plane),
             (__attribute__((address_space(3))) void*)(&bufA(buf)[s][wave * 64]), 16, 0, 0);
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(p.LB + ub + s * lb_plane),
-            (__attribute__((address_space(3))) void*)(&bufB(buf)[s][wave * 64]), 16, 0, 0);
+        // (eight waves: the first four stage B's 64 rows)
+        if (WM == 2 || wave < 4)
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(p.LB + ub + s * lb_plane),
+              (__attribute__((address_space(3))) void*)(&bufB(buf)[s][wave * 64]), 16, 0, 0);
       }
       if (lp < pend) {
         lk0 += CK;
@@ -802,15 +810,23 @@ __global__ __launch_bounds__(kThreads, I8 ? 2 : 1) void split_gemm_kernel(SplitP
 
 template <int S, bool I8>
 int launch_split_gemm(tamx_context* ctx, tamx_worklist* wl, const SplitParams& prm,
-                      hipStream_t s, bool m32 = false) {
+                      hipStream_t s, bool m32 = false, bool rows128 = false) {
   auto kern = split_gemm_kernel<S, I8>;
-  if constexpr (I8)
+  int threads = kThreads, bm = SBM;
+  if constexpr (I8) {
     if (m32) kern = split_gemm_kernel<S, I8, true>;
+    if constexpr (S > 2)
+      if (rows128) {
+        kern = split_gemm_kernel<S, I8, false, 4>;
+        threads = 2 * kThreads;
+        bm = 2 * SBM;
+      }
+  }
   int occ = 0;
-  TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kThreads, 0));
+  TAMX_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, 0));
   if (occ < 1) occ = 1;
-  if (int rc = tamx::plan_items(wl, SBM, SBN, s)) return rc;
-  kern<<<ctx->num_cus * occ, kThreads, 0, s>>>(prm);
+  if (int rc = tamx::plan_items(wl, bm, SBN, s)) return rc;
+  kern<<<ctx->num_cus * occ, threads, 0, s>>>(prm);
   TAMX_LAUNCH_CHECK();
   return TAMX_OK;
 }
@@ -1388,6 +1404,11 @@ int tamx_grouped_gemm_split_scatter(tamx_context* ctx, tamx_worklist* wl, double
       return launch_split_gemm_i8_2w(a_plan->nslices, ctx, wl, prm, s);
     if (cfg && cfg->bm == SBM && cfg->bn == SBN && cfg->bk == SBK8 / 2)
       return launch_split_gemm_i8_ring(a_plan->nslices, ctx, wl, prm, s);
+    // 128 x 64 from five limbs on: eight waves of 32 x 32 (the wide
+    // kernel's 64 x 32 waves need too many accumulator registers there)
+    if (cfg && cfg->bm == 2 * SBM && cfg->bn == SBN && a_plan->nslices >= 5)
+      return a_plan->nslices == 5 ? launch_split_gemm<5, true>(ctx, wl, prm, s, false, true)
+                                  : launch_split_gemm<6, true>(ctx, wl, prm, s, false, true);
     if (cfg && !(cfg->bm == SBM && cfg->bn == SBN))
       return launch_split_gemm_wide(TAMX_LIMB_I8, a_plan->nslices, cfg->bm, cfg->bn, ctx, wl,
                                     prm, s);
