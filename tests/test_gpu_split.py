@@ -670,13 +670,13 @@ def test_split_i8_autotune_and_wide_tile_rejected(ctx):
     torch.cuda.synchronize()
     tile, ms = c.tune(reps=1, force=True)
     assert tile[:2] in ((64, 64), (128, 64), (64, 128)) and ms > 0.0
-    # five INT8 limbs have the 64 x 64 tile only
+    # five INT8 limbs have 64 x 64 and the eight-wave 128 x 64, not 64 x 128
     c = E.Contraction(ctx, "m,n", A, "m,k", B, "k,n", threshold=1e-8, precision="i8x5")
     c.run()
     torch.cuda.synchronize()
     tile, ms = c.tune(reps=1, force=True)
-    assert tile[:2] == (64, 64)
-    c.set_tile((128, 64))
+    assert tile[:2] in ((64, 64), (128, 64)) and ms > 0.0
+    c.set_tile((64, 128))
     with pytest.raises(_lib.TamxError) as err:
         c.gemm_step()
     assert err.value.status == _lib.TAMX_ERR_UNSUPPORTED

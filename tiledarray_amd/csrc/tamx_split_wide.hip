@@ -356,6 +356,6 @@ int tamx::split::launch_split_gemm_wide(int limb_type, int nslices, int bm, int 
   TAMX_WIDE(true, 4, 64, 128)
 #undef TAMX_WIDE
   tamx::set_error("split GEMM macro tile: 64x64 (any limb count), 128x64 or 64x128 (2 to 4 "
-                  "BF16 or 3 to 4 INT8 limbs)");
+                  "BF16 or 3 to 4 INT8 limbs), 128x64 (5 or 6 INT8 limbs)");
   return TAMX_ERR_UNSUPPORTED;
 }
