@@ -343,6 +343,7 @@ def main():
                                                        else [64, 64, 16]),
             "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
             "seed": info["seed"],
+            "generator": info["generator"],
         },
         "roofline": ({
             "bound": "mfma",
@@ -518,6 +519,7 @@ def bench_ladder(args):
                      "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
                      "tile_pairs": cnt["npairs"], "flops_per_step": flops,
                      "precision": main_prec, "seed": info["seed"],
+                     "generator": info["generator"],
                      "v_permute": "copy" if hasattr(c.B, "_perm_plan") else "index map"},
                     ({"bound": "mfma", "achieved": round(achieved, 3),
                       "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
@@ -726,7 +728,7 @@ def bench_tot(args):
                     {"result_cells": cnt["ntasks"], "cell_pairs": cnt["npairs"],
                      "flops_per_step": flops, "macro_tile": used_tile,
                      "tile_choice": "autotuner" if args.tile == "auto" else "fixed",
-                     "seed": w["seed"]},
+                     "seed": w["seed"], "generator": w["generator"]},
                     {"bound": "mfma", "achieved": round(flops / (gemm_ms * 1e-3) / 1e12, 3),
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "traffic": None,

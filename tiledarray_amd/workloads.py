@@ -1,9 +1,12 @@
 """Synthetic inputs of the contraction benchmarks, generated on the device.
 
 Tilings and sparsity masks come from numpy's PCG64 seeded per workload (so
-they are identical on every machine); tile data comes from torch's device
-generator seeded the same way.  Each builder returns DistArrays resident in
-HBM plus a dict describing what was generated.
+they are identical on every machine).  Tile data comes from a counter-based
+generator keyed by (base seed, workload, tensor, tile ordinal) and counted
+by the element's ordinal inside its tile (SplitMix64; `keyed_fill`), so a
+tile's values do not depend on where it lies in the pool, on the other
+tiles, or on the machine.  Each builder returns DistArrays resident in HBM
+plus a dict describing what was generated.
 
 Workloads (the benchmark drivers of examples/gemm/ta_dense.cpp,
 ta_sparse.cpp and ta_cc_abcd.cpp set the pattern):
@@ -37,8 +40,98 @@ def random_tiling(extent, lo, hi, rng):
     return TiledRange1(b)
 
 
-def _fill_uniform(pool, gen):
-    pool.uniform_(-1.0, 1.0, generator=gen)
+# workload ids in the generator key
+CFG_DENSE, CFG_BLOCKSPARSE, CFG_LADDER, CFG_TOT = 1, 2, 3, 4
+GENERATOR = ("SplitMix64 (Steele, Lea, Flood 2014) keyed by (seed, workload, tensor, "
+             "tile ordinal), counter = element ordinal in the tile")
+
+_M64 = (1 << 64) - 1
+_GAMMA = 0x9E3779B97F4A7C15
+
+
+def _s64(c):
+    """An unsigned 64-bit constant as the int64 with the same bits."""
+    c &= _M64
+    return c - (1 << 64) if c >= (1 << 63) else c
+
+
+def _mix_int(z):
+    """SplitMix64's output function on a Python int (mod 2^64)."""
+    z &= _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def _srl(x, k):
+    """Logical right shift of int64 tensor bits."""
+    return (x >> k) & ((1 << (64 - k)) - 1)
+
+
+def mix64(z):
+    """SplitMix64's output function on an int64 tensor (two's-complement
+    wraparound gives the unsigned arithmetic)."""
+    z = (z ^ _srl(z, 30)) * _s64(0xBF58476D1CE4E5B9)
+    z = (z ^ _srl(z, 27)) * _s64(0x94D049BB133111EB)
+    return z ^ _srl(z, 31)
+
+
+def tile_key(seed, config, tensor, tile):
+    """The generator state of one tile, as an unsigned int."""
+    k = _mix_int(_mix_int(_mix_int(seed) ^ config) ^ tensor)
+    return _mix_int((k + (tile + 1) * _GAMMA) & _M64)
+
+
+def keyed_fill(pool, offsets, volumes, config, tensor, seed=BASE_SEED, kind="uniform",
+               batch=1 << 26):
+    """Fills the present tiles of `pool` (offsets < 0: absent): element e of
+    tile t gets draw number e of the stream keyed by (seed, config, tensor,
+    t), draw i being mix64(key + (i + 1) * golden gamma).  kind: "uniform"
+    U[-1, 1) from the top 53 bits, "normal" N(0, 1) by Box-Muller from draws
+    2e and 2e + 1, "int" integers in [-8, 8]."""
+    offsets = np.asarray(offsets, np.int64)
+    volumes = np.asarray(volumes, np.int64)
+    present = np.nonzero((offsets >= 0) & (volumes > 0))[0]
+    order = present[np.argsort(offsets[present], kind="stable")]
+    dev = pool.device
+    g = _s64(_GAMMA)
+
+    def run(ts):
+        t = np.asarray(ts, np.int64)
+        o0 = int(offsets[t[0]])
+        vol = torch.as_tensor(volumes[t], device=dev)
+        n = int(volumes[t].sum())
+        keys = torch.as_tensor([_s64(tile_key(seed, config, tensor, int(x))) for x in t],
+                               dtype=torch.int64, device=dev)
+        starts = torch.as_tensor(offsets[t] - o0, device=dev)
+        e = torch.arange(n, dtype=torch.int64, device=dev) - torch.repeat_interleave(starts, vol)
+        k = torch.repeat_interleave(keys, vol)
+        out = pool[o0:o0 + n]
+        if kind == "normal":
+            z1 = mix64(k + (2 * e + 1) * g)
+            z2 = mix64(k + (2 * e + 2) * g)
+            u1 = (_srl(z1, 11) + 1).to(torch.float64) * 2.0 ** -53  # (0, 1]
+            u2 = _srl(z2, 11).to(torch.float64) * 2.0 ** -53
+            out.copy_(torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * math.pi * u2))
+        else:
+            z = mix64(k + (e + 1) * g)
+            if kind == "int":
+                out.copy_((_srl(z, 11) % 17 - 8).to(pool.dtype))
+            else:
+                out.copy_(_srl(z, 11).to(torch.float64) * 2.0 ** -52 - 1.0)
+
+    # runs of tiles that lie back to back in the pool, a batch of elements
+    # at a time
+    run_t, run_n = [], 0
+    for t in order:
+        contiguous = run_t and offsets[run_t[-1]] + volumes[run_t[-1]] == offsets[t]
+        if run_t and (not contiguous or run_n + volumes[t] > batch):
+            run(run_t)
+            run_t, run_n = [], 0
+        run_t.append(int(t))
+        run_n += int(volumes[t])
+    if run_t:
+        run(run_t)
 
 
 def _tile_scale_(arr, scales):
@@ -60,16 +153,12 @@ def _tile_scale_(arr, scales):
 def dense_gemm(n=2048, tile=128, seed=BASE_SEED, dtype=torch.float64, device="cuda",
                integer=False):
     tr = TiledRange1.uniform(n, tile)
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
     A = DistArray.allocate(TiledRange([tr, tr]), dtype=dtype, device=device)
     B = DistArray.allocate(TiledRange([tr, tr]), dtype=dtype, device=device)
-    for X in (A, B):
-        if integer:
-            X.pool.copy_(torch.randint(-8, 9, X.pool.shape, generator=gen, device=device))
-        else:
-            _fill_uniform(X.pool, gen)
-    return A, B, {"M": n, "N": n, "K": n, "tile": tile, "seed": seed}
+    for w, X in enumerate((A, B)):
+        keyed_fill(X.pool, X.offsets, X.trange.tile_volumes(), CFG_DENSE, w, seed,
+                   "int" if integer else "uniform")
+    return A, B, {"M": n, "N": n, "K": n, "tile": tile, "seed": seed, "generator": GENERATOR}
 
 
 def blocksparse_gemm(n=16384, wlo=64, whi=256, band=8.0, decay=4.0, seed=BASE_SEED,
@@ -82,19 +171,17 @@ def blocksparse_gemm(n=16384, wlo=64, whi=256, band=8.0, decay=4.0, seed=BASE_SE
     prob = np.exp(-dist / band)
     mask_a = rng.random((nt, nt)) < prob
     mask_b = rng.random((nt, nt)) < prob
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
     A = DistArray.allocate(TiledRange([tr, tr]), mask_a, dtype=dtype, device=device)
     B = DistArray.allocate(TiledRange([tr, tr]), mask_b, dtype=dtype, device=device)
     scale = 10.0 ** (-dist.reshape(-1) / decay)
-    for X in (A, B):
-        if integer:
-            X.pool.copy_(torch.randint(-8, 9, X.pool.shape, generator=gen, device=device))
-        else:
-            _fill_uniform(X.pool, gen)
+    for w, X in enumerate((A, B)):
+        keyed_fill(X.pool, X.offsets, X.trange.tile_volumes(), CFG_BLOCKSPARSE, w, seed,
+                   "int" if integer else "uniform")
+        if not integer:
             _tile_scale_(X, scale)
     info = {"M": n, "N": n, "K": n, "tiles_per_mode": nt, "widths": [wlo, whi],
-            "fill_a": float(mask_a.mean()), "fill_b": float(mask_b.mean()), "seed": seed}
+            "fill_a": float(mask_a.mean()), "fill_b": float(mask_b.mean()), "seed": seed,
+            "generator": GENERATOR}
     return A, B, info
 
 
@@ -105,13 +192,12 @@ def ladder(o=32, v=256, otiles=2, wlo=48, whi=96, seed=BASE_SEED, device="cuda",
     rng = np.random.default_rng(seed)
     trv = random_tiling(v, wlo, whi, rng)
     tro = TiledRange1.uniform(o, (o + otiles - 1) // otiles)
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
     T = DistArray.allocate(TiledRange([trv, trv, tro, tro]), dtype=dtype, device=device)
-    T.pool.normal_(0.0, math.sqrt(1e-2), generator=gen)
+    keyed_fill(T.pool, T.offsets, T.trange.tile_volumes(), CFG_LADDER, 0, seed, "normal")
+    T.pool.mul_(math.sqrt(1e-2))
     modes = {"a": trv, "b": trv, "c": trv, "d": trv}
     V = DistArray.allocate(TiledRange([modes[x] for x in v_order]), dtype=dtype, device=device)
-    V.pool.normal_(0.0, 1.0, generator=gen)
+    keyed_fill(V.pool, V.offsets, V.trange.tile_volumes(), CFG_LADDER, 1, seed, "normal")
     # exp(-|a-c|/64) on every element: per tile, a broadcast over the other modes
     pa, pc = v_order.index("a"), v_order.index("c")
     ext = V.trange.tile_extents_table()
@@ -135,7 +221,7 @@ def ladder(o=32, v=256, otiles=2, wlo=48, whi=96, seed=BASE_SEED, device="cuda",
         fv = f if pa < pc else f.t()
         tile.mul_(fv.reshape(shape))
     info = {"o": o, "v": v, "v_tiles": trv.ntiles, "v_widths": trv.extents.tolist(),
-            "T_sigma2": 1e-2, "v_order": v_order, "seed": seed}
+            "T_sigma2": 1e-2, "v_order": v_order, "seed": seed, "generator": GENERATOR}
     return T, V, info
 
 
@@ -165,15 +251,11 @@ def tot_batch(nbatch=1024, grid=8, fill=0.5, lo=16, hi=96, seed=BASE_SEED, dtype
 
     a_off, na = offsets(ma, va)
     b_off, nb = offsets(mb, vb)
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
-    A = torch.empty(max(1, na), dtype=dtype, device=device)
-    B = torch.empty(max(1, nb), dtype=dtype, device=device)
-    for X in (A, B):
-        if integer:
-            X.copy_(torch.randint(-8, 9, X.shape, generator=gen, device=device))
-        else:
-            X.uniform_(-1.0, 1.0, generator=gen)
+    A = torch.zeros(max(1, na), dtype=dtype, device=device)
+    B = torch.zeros(max(1, nb), dtype=dtype, device=device)
+    for w, (X, off, vol) in enumerate(((A, a_off, va), (B, b_off, vb))):
+        keyed_fill(X, off, vol, CFG_TOT, w, seed, "int" if integer else "uniform")
     return {"nbatch": nbatch, "grid": g, "a_dims": ad, "c_dims": cd, "b_dims": bd,
             "a_off": a_off, "b_off": b_off, "a_vol": va, "b_vol": vb, "A": A, "B": B,
-            "fill_a": float(ma.mean()), "fill_b": float(mb.mean()), "seed": seed}
+            "fill_a": float(ma.mean()), "fill_b": float(mb.mean()), "seed": seed,
+            "generator": GENERATOR}
