@@ -79,6 +79,22 @@ def test_fused_result_permute_equals_permute_pass(ctx, target, precision):
         assert rel_fro(Rf.to_dense(), ref) <= (1e-6 if precision != "bf16x3" else 1e-5)
 
 
+@pytest.mark.parametrize("target", ["b,a,i,j", "j,b,a,i"])
+def test_fused_result_permute_eight_wave_split(ctx, target):
+    # the eight-wave INT8 kernel (128 x 64 macro tiles, five limbs) writes
+    # through the same scatter map as the 64 x 64 kernel: equal results
+    T, V = _integer_ladder(13)
+    out = []
+    for tile in (None, (128, 64)):
+        c = E.Contraction(ctx, target, T, "c,d,i,j", V, "a,b,c,d", precision="i8x5", tile=tile)
+        assert c.c_scatter is not None
+        c.run()
+        out.append(_no_sync_result(c))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out[0].offsets, out[1].offsets)
+    assert torch.equal(out[0].pool, out[1].pool)
+
+
 def test_fused_result_permute_fp32_and_alpha(ctx):
     T, V = _integer_ladder(12)
     T32 = DistArray(T.trange, T.pool.float(), T.offsets)
