@@ -123,7 +123,9 @@ int tamx_worklist_create(tamx_context* ctx, int64_t task_capacity,
   wl->item_capacity = std::min<int64_t>(tc * 16, (int64_t(1) << 31) - 1);
   alloc(&wl->item_task, sizeof(int32_t) * wl->item_capacity);
   alloc(&wl->queue, sizeof(int32_t) * 16);
-  const int64_t nparts = std::max<int64_t>(tamx::kScanBlocks, gc / 256 + 1) + 1;
+  // one partial per work-list workgroup (32 result tiles each), at least
+  // one per item-planner chunk
+  const int64_t nparts = std::max<int64_t>(tamx::kScanBlocks, gc / 32 + 1) + 1;
   alloc(reinterpret_cast<char**>(&wl->scan_partials), 32 * nparts);
   if (e == hipSuccess) e = hipMemset(wl->counters, 0, sizeof(int64_t) * TAMX_WL_NCOUNTERS);
   if (e == hipSuccess) e = hipMemset(wl->queue, 0, sizeof(int32_t) * 16);

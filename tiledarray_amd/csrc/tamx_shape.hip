@@ -380,7 +380,11 @@ struct Quad {
   }
 };
 
-constexpr int kWlThreads = 256;  // result tiles per workgroup of count / fill
+constexpr int kWlThreads = 256;  // threads per workgroup of count / fill
+// lanes that share one result tile's k walk (each takes a contiguous k
+// range), and so result tiles per workgroup
+constexpr int kWlSub = 8;
+constexpr int kWlTiles = kWlThreads / kWlSub;
 
 struct WlParams {
   tamx_contraction_desc d;
@@ -395,28 +399,35 @@ struct WlParams {
   int64_t task_capacity, pair_capacity;
 };
 
-__device__ inline bool b_present(const tamx_contraction_desc& d, int64_t bi) {
-  return d.b_off[bi] >= 0 && (!d.b_norms || !(d.b_norms[bi] < d.b_threshold));
-}
-// The A side of a pair (offset and norm of A_ik), read one k ahead of its
-// use by the work-list loops so that the next load is in flight while the
-// current k is tested (most A_ik are absent in a banded pattern, and the
-// test of an absent one waits only on this load).
-struct ASide {
-  int64_t off;
-  float norm;
+// The work-list walks load the operands' offsets and norms for kWlK
+// consecutive k at once (every load in flight before any test; a k past the
+// end reads the last one again and is not tested), then test the pairs in k
+// order: (A_ik, B_kj) contributes when both are stored and not screened out
+// and (optional per-pair rule) |A_ik| |B_kj| >= thr_k.
+constexpr int kWlK = 8;
+struct KSides {
+  int64_t a_off[kWlK], b_off[kWlK];
+  float a_norm[kWlK], b_norm[kWlK];
 };
-__device__ inline ASide a_side(const tamx_contraction_desc& d, int64_t ai) {
-  return {d.a_off[ai], d.a_norms ? d.a_norms[ai] : 0.0f};
+template <class AiOf, class BiOf>
+__device__ inline void load_k_sides(const tamx_contraction_desc& d, int64_t k0, AiOf ai_of,
+                                    BiOf bi_of, KSides& s) {
+#pragma unroll
+  for (int u = 0; u < kWlK; ++u) {
+    const int64_t k = min<int64_t>(k0 + u, d.Kt - 1);
+    const int64_t ai = ai_of(k), bi = bi_of(k);
+    s.a_off[u] = d.a_off[ai];
+    s.b_off[u] = d.b_off[bi];
+    s.a_norm[u] = d.a_norms ? d.a_norms[ai] : 0.0f;
+    s.b_norm[u] = d.b_norms ? d.b_norms[bi] : 0.0f;
+  }
 }
-// Whether (A_ik, B_kj) contributes: both stored and not screened out, and
-// (optional per-pair rule) |A_ik| |B_kj| >= thr_k.
-__device__ inline bool pair_kept_a(const tamx_contraction_desc& d, const ASide& a, int64_t bi,
+__device__ inline bool pair_kept_k(const tamx_contraction_desc& d, const KSides& s, int u,
                                    float thr_k) {
 #pragma clang fp contract(off)
-  if (!(a.off >= 0 && (!d.a_norms || !(a.norm < d.a_threshold)))) return false;
-  if (!b_present(d, bi)) return false;
-  if (thr_k > 0.0f && d.a_norms && d.b_norms && a.norm * d.b_norms[bi] < thr_k) return false;
+  if (!(s.a_off[u] >= 0 && (!d.a_norms || !(s.a_norm[u] < d.a_threshold)))) return false;
+  if (!(s.b_off[u] >= 0 && (!d.b_norms || !(s.b_norm[u] < d.b_threshold)))) return false;
+  if (thr_k > 0.0f && d.a_norms && d.b_norms && s.a_norm[u] * s.b_norm[u] < thr_k) return false;
   return true;
 }
 
@@ -425,13 +436,31 @@ __device__ inline Quad tile_quad(const WlParams& p, int64_t g) {
   return Quad(v > 0 ? 1 : 0, p.tile_npairs[g], v, p.tile_flops[g]);
 }
 
+// k range [kb, ke) of sub-walker `sub` of a tile's walk
+__device__ inline void sub_range(int64_t Kt, int sub, int64_t& kb, int64_t& ke) {
+  const int64_t chunk = (Kt + kWlSub - 1) / kWlSub;
+  kb = min<int64_t>(Kt, sub * chunk);
+  ke = min<int64_t>(Kt, kb + chunk);
+}
+
+// Sum over the kWlSub consecutive lanes that walk one tile (the lanes of a
+// group are aligned inside the wave).
+template <typename V>
+__device__ inline V sub_sum(V v) {
+#pragma unroll
+  for (int o = 1; o < kWlSub; o <<= 1) v += __shfl_xor(v, o, kWlSub);
+  return v;
+}
+
 // Pass 1: per result tile, the number of contributing pairs, the element
 // count and the flops (zero for result tiles screened out by the shape);
-// per workgroup, the sum of its tiles' Quads.
+// per workgroup, the sum of its tiles' Quads.  kWlSub lanes share a tile,
+// each walking a contiguous range of k.
 __global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
 #pragma clang fp contract(off)
   const tamx_contraction_desc& d = p.d;
-  const int64_t g = (int64_t)blockIdx.x * kWlThreads + threadIdx.x;
+  const int sub = threadIdx.x % kWlSub;
+  const int64_t g = (int64_t)blockIdx.x * kWlTiles + threadIdx.x / kWlSub;
   const int64_t per = d.Mt * d.Nt;
   int32_t np = 0;
   int64_t vol = 0, flops = 0;
@@ -446,17 +475,27 @@ __global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
       auto ai_of = [&](int64_t k) {
         return b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
       };
-      ASide next = d.Kt > 0 ? a_side(d, ai_of(0)) : ASide{-1, 0.0f};
-      for (int64_t k = 0; k < d.Kt; ++k) {
-        const ASide a = next;
-        if (k + 1 < d.Kt) next = a_side(d, ai_of(k + 1));
-        const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
-        if (pair_kept_a(d, a, bi, thr_k)) {
-          ++np;
-          flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k];
-        }
+      auto bi_of = [&](int64_t k) {
+        return b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
+      };
+      int64_t kb, ke;
+      sub_range(d.Kt, sub, kb, ke);
+      for (int64_t k0 = kb; k0 < ke; k0 += kWlK) {
+        KSides sd;
+        load_k_sides(d, k0, ai_of, bi_of, sd);
+#pragma unroll
+        for (int u = 0; u < kWlK; ++u)
+          if (k0 + u < ke && pair_kept_k(d, sd, u, thr_k)) {
+            ++np;
+            flops += 2 * m * n * (int64_t)d.kext[b * d.Kt + k0 + u];
+          }
       }
     }
+  }
+  np = sub_sum(np);
+  flops = sub_sum(flops);
+  if (sub != 0) vol = 0;  // the tile counts once in the workgroup's sum
+  if (sub == 0 && g < per * d.nbatch) {
     p.tile_npairs[g] = np;
     p.tile_volume[g] = vol;
     p.tile_flops[g] = flops;
@@ -464,7 +503,7 @@ __global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
   // raw storage: __shared__ objects cannot have constructors
   __shared__ __align__(16) unsigned char red_raw[sizeof(Quad) * kWlThreads];
   Quad* red = reinterpret_cast<Quad*>(red_raw);
-  red[threadIdx.x] = Quad(vol > 0 ? 1 : 0, np, vol, flops);
+  red[threadIdx.x] = sub == 0 ? Quad(vol > 0 ? 1 : 0, np, vol, flops) : Quad(0);
   __syncthreads();
   for (int w = kWlThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -473,8 +512,6 @@ __global__ __launch_bounds__(kWlThreads) void wl_count_kernel(WlParams p) {
   if (threadIdx.x == 0) p.partial[blockIdx.x] = red[0];
 }
 
-// Pass 2 (one workgroup): exclusive scan of the per-workgroup sums, the
-// totals and the capacity check.
 __global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p, int nparts) {
   __shared__ __align__(16) unsigned char part_raw[sizeof(Quad) * 1024];
   Quad* part = reinterpret_cast<Quad*>(part_raw);
@@ -498,69 +535,102 @@ __global__ __launch_bounds__(1024) void wl_scan_kernel(WlParams p, int nparts) {
   }
 }
 
-// Pass 3: each workgroup rescans its own 256 tiles from its offset and
-// writes tasks (row-major tile order), pairs (k ascending) and C offsets.
+// Pass 3: every workgroup rescans its tiles from its offset and writes the
+// tasks (row-major tile order), the C offsets and the pairs (k ascending):
+// the kWlSub lanes of a tile count their k ranges, scan the counts, and
+// write their pairs in k order.
 __global__ __launch_bounds__(kWlThreads) void wl_fill_kernel(WlParams p) {
 #pragma clang fp contract(off)
   const tamx_contraction_desc& d = p.d;
   const int64_t per = d.Mt * d.Nt;
-  const int64_t first = (int64_t)blockIdx.x * kWlThreads;
-  const int64_t g = first + threadIdx.x;
-  const int64_t nloc = min<int64_t>(kWlThreads, per * d.nbatch - first);
+  const int64_t first = (int64_t)blockIdx.x * kWlTiles;
+  const int sub = threadIdx.x % kWlSub;
+  const int lt = threadIdx.x / kWlSub;  // tile of the workgroup
+  const int64_t g = first + lt;
+  const int64_t nloc = min<int64_t>(kWlTiles, per * d.nbatch - first);
   __shared__ __align__(16) unsigned char part_raw[sizeof(Quad) * kWlThreads];
+  __shared__ __align__(16) unsigned char mine_raw[sizeof(Quad) * kWlTiles];
   Quad* part = reinterpret_cast<Quad*>(part_raw);
-  Quad mine(0);
-  // one tile per thread: thread t receives the prefix of tile first + t
+  Quad* mine_l = reinterpret_cast<Quad*>(mine_raw);
+  // thread t < nloc receives the prefix of tile first + t
   tamx::block_exclusive_scan<kWlThreads, Quad>(
       nloc, part, [&](int64_t i) -> Quad { return tile_quad(p, first + i); },
-      [&](int64_t, const Quad& v) { mine = v; });
+      [&](int64_t i, const Quad& v) {
+        Quad x = p.partial[blockIdx.x];
+        x += v;
+        mine_l[i] = x;
+      });
+  __syncthreads();
   if (g >= per * d.nbatch) return;
-  mine += p.partial[blockIdx.x];
   const bool ok = p.counters[TAMX_WL_STATUS] == 0;
   if (!ok || p.tile_volume[g] == 0) {
-    d.c_off[g] = -1;
+    if (sub == 0) d.c_off[g] = -1;
     return;
   }
+  const Quad mine = mine_l[lt];
   const int64_t b = g / per, ij = g % per, i = ij / d.Nt, j = ij % d.Nt;
   const int32_t m = d.mext[b * d.Mt + i], n = d.next[b * d.Nt + j];
-  const int64_t coff = mine.c;
-  d.c_off[g] = coff;
   const int32_t t = (int32_t)mine.t;
   const int32_t pb = (int32_t)mine.p;
-  tamx_gemm_task task;
-  task.c_off = coff;
-  task.m = m;
-  task.n = n;
-  task.ldc = d.store_trans ? m : n;
-  task.pair_begin = pb;
-  task.pair_count = p.tile_npairs[g];
-  task.flags = d.store_trans ? TAMX_TASK_STORE_TRANS : 0;
-  p.tasks[t] = task;
-  p.task_tile[t] = (int32_t)g;
+  if (sub == 0) {
+    const int64_t coff = mine.c;
+    d.c_off[g] = coff;
+    tamx_gemm_task task;
+    task.c_off = coff;
+    task.m = m;
+    task.n = n;
+    task.ldc = d.store_trans ? m : n;
+    task.pair_begin = pb;
+    task.pair_count = p.tile_npairs[g];
+    task.flags = d.store_trans ? TAMX_TASK_STORE_TRANS : 0;
+    p.tasks[t] = task;
+    p.task_tile[t] = (int32_t)g;
+  }
   const float thr_k =
       d.pair_threshold > 0.0f ? d.pair_threshold / (float)d.Kt : 0.0f;
-  int32_t q = pb;
   auto ai_of = [&](int64_t k) {
     return b * d.Mt * d.Kt + (d.op_a ? k * d.Mt + i : i * d.Kt + k);
   };
-  ASide next = d.Kt > 0 ? a_side(d, ai_of(0)) : ASide{-1, 0.0f};
-  for (int64_t k = 0; k < d.Kt; ++k) {
-    const ASide a = next;
-    if (k + 1 < d.Kt) next = a_side(d, ai_of(k + 1));
-    const int64_t ai = ai_of(k);
-    const int64_t bi = b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
-    if (!pair_kept_a(d, a, bi, thr_k)) continue;
-    const int32_t kk = d.kext[b * d.Kt + k];
-    tamx_gemm_pair pr;
-    pr.a_off = a.off;
-    pr.b_off = d.b_off[bi];
-    pr.k = kk;
-    pr.lda = d.op_a ? m : kk;
-    pr.ldb = d.op_b ? kk : n;
-    pr.a_tile = (int32_t)ai;
-    pr.b_tile = (int32_t)bi;
-    pr.reserved = 0;
-    p.pairs[q++] = pr;
+  auto bi_of = [&](int64_t k) {
+    return b * d.Kt * d.Nt + (d.op_b ? j * d.Kt + k : k * d.Nt + j);
+  };
+  int64_t kb, ke;
+  sub_range(d.Kt, sub, kb, ke);
+  // this lane's pairs, then its start among the tile's pairs
+  int32_t cnt = 0;
+  for (int64_t k0 = kb; k0 < ke; k0 += kWlK) {
+    KSides sd;
+    load_k_sides(d, k0, ai_of, bi_of, sd);
+#pragma unroll
+    for (int u = 0; u < kWlK; ++u)
+      if (k0 + u < ke && pair_kept_k(d, sd, u, thr_k)) ++cnt;
+  }
+  int32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < kWlSub; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o, kWlSub);
+    if (sub >= o) incl += y;
+  }
+  int32_t q = pb + incl - cnt;
+  for (int64_t k0 = kb; k0 < ke; k0 += kWlK) {
+    KSides sd;
+    load_k_sides(d, k0, ai_of, bi_of, sd);
+#pragma unroll
+    for (int u = 0; u < kWlK; ++u) {
+      const int64_t k = k0 + u;
+      if (!(k < ke && pair_kept_k(d, sd, u, thr_k))) continue;
+      const int32_t kk = d.kext[b * d.Kt + k];
+      tamx_gemm_pair pr;
+      pr.a_off = sd.a_off[u];
+      pr.b_off = sd.b_off[u];
+      pr.k = kk;
+      pr.lda = d.op_a ? m : kk;
+      pr.ldb = d.op_b ? kk : n;
+      pr.a_tile = (int32_t)ai_of(k);
+      pr.b_tile = (int32_t)bi_of(k);
+      pr.reserved = 0;
+      p.pairs[q++] = pr;
+    }
   }
 }
 
@@ -860,7 +930,7 @@ int tamx_contraction_worklist(tamx_context* ctx,
              wl->task_capacity,
              wl->pair_capacity};
   hipStream_t s = tamx::as_stream(stream);
-  const unsigned g = (unsigned)std::max<int64_t>(1, (grid + kWlThreads - 1) / kWlThreads);
+  const unsigned g = (unsigned)std::max<int64_t>(1, (grid + kWlTiles - 1) / kWlTiles);
   wl_count_kernel<<<g, kWlThreads, 0, s>>>(p);
   TAMX_LAUNCH_CHECK();
   wl_scan_kernel<<<1, 1024, 0, s>>>(p, (int)g);
