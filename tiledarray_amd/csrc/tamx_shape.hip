@@ -13,6 +13,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -133,14 +134,23 @@ __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(
 template <typename T>
 __device__ inline float finish_norm(const double* __restrict__ partial, int64_t b, int64_t e) {
   if (b == e) return 0.0f;
-  if (sizeof(T) == sizeof(double)) {
-    double s = 0.0;
-    for (int64_t c = b; c < e; ++c) s += partial[c];
-    return (float)sqrt(s);
+  // the chunks are added one after another, in order; sixteen are loaded
+  // before they are added, so a tile of many chunks waits for a load
+  // sixteen times less often
+  constexpr int U = 16;
+  using acc_t = std::conditional_t<sizeof(T) == sizeof(double), double, float>;
+  acc_t s = 0;
+  int64_t c = b;
+  for (; c + U <= e; c += U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = partial[c + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += (acc_t)v[u];
   }
-  float s = 0.0f;
-  for (int64_t c = b; c < e; ++c) s += (float)partial[c];
-  return sqrtf(s);
+  for (; c < e; ++c) s += (acc_t)partial[c];
+  if constexpr (sizeof(T) == sizeof(double)) return (float)sqrt(s);
+  else return sqrtf(s);
 }
 
 // Combines the chunks of each tile (finish_norm); with maxabs, also the
@@ -156,7 +166,15 @@ __global__ void norm_finish_kernel(const int64_t* __restrict__ tchunk,
   const int64_t b = tchunk[t], e = tchunk[t + 1];
   if (maxabs) {
     double m = 0.0;
-    for (int64_t c = b; c < e; ++c) m = fmax(m, partial_max[c]);
+    int64_t c = b;
+    for (; c + 16 <= e; c += 16) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = partial_max[c + u];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) m = fmax(m, v[u]);
+    }
+    for (; c < e; ++c) m = fmax(m, partial_max[c]);
     maxabs[t] = m;
   }
   norms[t] = finish_norm<T>(partial, b, e);
